@@ -1,0 +1,258 @@
+// extern "C" entry points declared in include/edgeto_amd.h.  Each one
+// validates its arguments, converts them to the internal Grid description and
+// forwards to the kernels; no exception crosses this boundary.
+#include <new>
+#include <string>
+
+#include "blas.hpp"
+#include "common.hpp"
+#include "solver.hpp"
+#include "stiffness.hpp"
+
+namespace et {
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+}  // namespace et
+
+using et::Grid;
+
+#define ET_GUARD(body)                                   \
+  try {                                                  \
+    body                                                 \
+  } catch (const std::bad_alloc&) {                      \
+    et::set_error("host allocation failed");             \
+    return ET_ENOMEM;                                    \
+  } catch (...) {                                        \
+    et::set_error("unexpected C++ exception");           \
+    return ET_EINVAL;                                    \
+  }
+
+extern "C" {
+
+const char* et_last_error(void) { return et::last_error(); }
+const char* et_version(void) { return "edgeto_amd 0.1 (gfx950)"; }
+
+int et_generate_elements(int dim, const int* nel, int32_t* elements, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(elements, "elements is null");
+    return et::generate_elements(g, elements, stream);
+  })
+}
+
+int et_matvec_f64(int dim, const int* nel, const double* K0, const double* E, const uint8_t* cons,
+                  const double* u, double* y, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(K0 && E && u && y, "et_matvec_f64: null pointer");
+    ET_REQUIRE(u != y, "et_matvec_f64: u and y must not alias");
+    return et::matvec<double>(g, K0, E, cons, u, y, stream);
+  })
+}
+
+int et_matvec_f32(int dim, const int* nel, const float* K0, const float* E, const uint8_t* cons,
+                  const float* u, float* y, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(K0 && E && u && y, "et_matvec_f32: null pointer");
+    ET_REQUIRE(u != y, "et_matvec_f32: u and y must not alias");
+    return et::matvec<float>(g, K0, E, cons, u, y, stream);
+  })
+}
+
+int et_diagonal_f64(int dim, const int* nel, const double* K0, const double* E,
+                    const uint8_t* cons, double* diag, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(K0 && E && diag, "et_diagonal_f64: null pointer");
+    return et::diagonal<double>(g, K0, E, cons, diag, stream);
+  })
+}
+
+int et_elem_energy_f64(int dim, const int* nel, const double* K0, const double* U, double* dk,
+                       hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(K0 && U && dk, "et_elem_energy_f64: null pointer");
+    return et::elem_energy(g, K0, U, nullptr, 0.0, 0.0, 0, dk, stream);
+  })
+}
+
+int et_sensitivity_f64(int dim, const int* nel, const double* K0, const double* U,
+                       const double* xt, double penal, double void_, double* out,
+                       hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(K0 && U && xt && out, "et_sensitivity_f64: null pointer");
+    return et::elem_energy(g, K0, U, xt, penal, void_, 1, out, stream);
+  })
+}
+
+int et_simp_forward_f64(int64_t n, const double* xt, double penal, double void_, int plain,
+                        double* E, hipStream_t stream) {
+  ET_GUARD({
+    ET_REQUIRE(n > 0 && xt && E, "et_simp_forward_f64: bad arguments");
+    return et::simp_forward(n, xt, penal, void_, plain, E, stream);
+  })
+}
+
+int et_filter_normalizer_f64(int dim, const int* nel, int n_off, const int32_t* offsets,
+                             const double* w, double* norm, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(offsets && w && norm, "et_filter_normalizer_f64: null pointer");
+    return et::filter_normalizer(g, n_off, offsets, w, norm, stream);
+  })
+}
+
+int et_filter_fwd_f64(int dim, const int* nel, int n_off, const int32_t* offsets, const double* w,
+                      const double* x, double* y, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(offsets && w && x && y, "et_filter_fwd_f64: null pointer");
+    ET_REQUIRE(x != y, "et_filter_fwd_f64: x and y must not alias");
+    return et::filter_fwd(g, n_off, offsets, w, x, y, stream);
+  })
+}
+
+int et_filter_adj_f64(int dim, const int* nel, int n_off, const int32_t* offsets, const double* w,
+                      const double* norm, const double* v, double* y, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(offsets && w && norm && v && y, "et_filter_adj_f64: null pointer");
+    ET_REQUIRE(v != y, "et_filter_adj_f64: v and y must not alias");
+    return et::filter_adj(g, n_off, offsets, w, norm, v, y, stream);
+  })
+}
+
+int et_restrict_f64(int dim, const int* nel, int dof, const double* vf, double* vc,
+                    hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(vf && vc && dof >= 1 && dof <= 3, "et_restrict_f64: bad arguments");
+    return et::restrict_op<double>(g, dof, vf, vc, stream);
+  })
+}
+
+int et_prolong_f64(int dim, const int* nel, int dof, const double* vc, double* vf,
+                   hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(vf && vc && dof >= 1 && dof <= 3, "et_prolong_f64: bad arguments");
+    return et::prolong_op<double>(g, dof, vc, vf, stream);
+  })
+}
+
+int et_stencil_from_operator_f64(int dim, const int* nel, const double* K0, const double* E,
+                                 const uint8_t* cons, double* S, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(K0 && E && S, "et_stencil_from_operator_f64: null pointer");
+    return et::stencil_from_operator(g, K0, E, cons, S, stream);
+  })
+}
+
+int et_galerkin_f64(int dim, const int* nel, int dof, const double* S_fine, double* S_coarse,
+                    hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(S_fine && S_coarse, "et_galerkin_f64: null pointer");
+    return et::galerkin(g, dof, S_fine, S_coarse, stream);
+  })
+}
+
+int et_stencil_apply_f64(int dim, const int* nel, int dof, const double* S, const double* x,
+                         double* y, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(S && x && y && dof == dim, "et_stencil_apply_f64: bad arguments");
+    return et::stencil_apply<double>(g, dof, S, x, y, stream);
+  })
+}
+
+int et_dot_f64(int64_t n, const double* a, const double* b, double* out, hipStream_t stream) {
+  ET_GUARD({
+    ET_REQUIRE(n > 0 && a && b && out, "et_dot_f64: bad arguments");
+    // partial sums live in a per-thread device scratch buffer kept for the process
+    static thread_local double* partial = nullptr;
+    if (partial == nullptr) ET_CHECK_HIP(hipMalloc(&partial, et::kRedBlocks * sizeof(double)));
+    return et::dot(n, a, b, partial, out, stream);
+  })
+}
+
+int et_sqdist_f64(int64_t n, const double* a, const double* b, double* out, hipStream_t stream) {
+  ET_GUARD({
+    ET_REQUIRE(n > 0 && a && b && out, "et_sqdist_f64: bad arguments");
+    static thread_local double* partial = nullptr;
+    if (partial == nullptr) ET_CHECK_HIP(hipMalloc(&partial, et::kRedBlocks * sizeof(double)));
+    return et::sqdist(n, a, b, partial, out, stream);
+  })
+}
+
+int64_t et_oc_work_size(int64_t n) { return et::oc_work_size(n); }
+
+int et_oc_update_f64(int64_t n, const double* x, const double* dc, double A, double vol, double vf,
+                     double move, double* x_new, void* work, int* steps, hipStream_t stream) {
+  ET_GUARD({ return et::oc_update(n, x, dc, A, vol, vf, move, x_new, work, steps, stream); })
+}
+
+int et_mg_create(et_mg** handle, const et_mg_params* params, const double* K0_host,
+                 hipStream_t stream) {
+  ET_GUARD({
+    ET_REQUIRE(handle && params && K0_host, "et_mg_create: null pointer");
+    *handle = nullptr;
+    et_mg* h = new et_mg();
+    const int st = h->impl.create(*params, K0_host, stream);
+    if (st != ET_OK) {
+      h->impl.destroy();
+      delete h;
+      return st;
+    }
+    *handle = h;
+    return ET_OK;
+  })
+}
+
+int et_mg_destroy(et_mg* handle) {
+  ET_GUARD({
+    if (handle) {
+      handle->impl.destroy();
+      delete handle;
+    }
+    return ET_OK;
+  })
+}
+
+int et_mg_solve(et_mg* handle, const double* E, const uint8_t* cons, const double* b, double* x,
+                double tol, int maxiter, int* iters, double* residual) {
+  ET_GUARD({
+    ET_REQUIRE(handle, "et_mg_solve: null handle");
+    return handle->impl.solve(E, cons, b, x, tol, maxiter, iters, residual);
+  })
+}
+
+int et_mg_last_timings(et_mg* handle, double* setup_ms, double* pcg_ms) {
+  ET_GUARD({
+    ET_REQUIRE(handle, "et_mg_last_timings: null handle");
+    if (setup_ms) *setup_ms = handle->impl.t_setup_ms;
+    if (pcg_ms) *pcg_ms = handle->impl.t_pcg_ms;
+    return ET_OK;
+  })
+}
+
+}  // extern "C"

@@ -1,0 +1,325 @@
+// Coarsest-grid direct solve in FP64: the coarsest Galerkin stencil is
+// expanded to a dense SPD matrix, factored by a blocked right-looking
+// Cholesky (64x64 tiles in LDS), and its inverse is formed as
+// A^-1 = L^-T L^-1 so that every V-cycle's coarse solve is one dense
+// symmetric matrix-vector product (one wavefront per row, coalesced).
+// The matrix is padded to a multiple of 64 with an identity block.
+#include "common.hpp"
+#include "coarse.hpp"
+
+namespace et {
+
+constexpr int NB = 64;
+
+// ------------------------------------------------ dense from block stencil ---
+
+template <int DIM, int DOF>
+__global__ void dense_from_stencil_kernel(const Grid g, const double* __restrict__ S, int ld,
+                                          double* __restrict__ A) {
+  constexpr int NO = DIM == 3 ? 27 : 9;
+  const int64_t N = g.n_nodes();
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= N * NO) return;
+  const int o = (int)(tid / N);
+  const int64_t n = tid % N;
+  const int oi = DIM == 3 ? o / 9 - 1 : o / 3 - 1;
+  const int oj = DIM == 3 ? (o / 3) % 3 - 1 : o % 3 - 1;
+  const int ok = DIM == 3 ? o % 3 - 1 : 0;
+  const int nz = g.nz(), nx = g.nx();
+  const int k = DIM == 3 ? (int)(n % nz) : 0;
+  const int64_t t = DIM == 3 ? n / nz : n;
+  const int i = (int)(t % nx), j = (int)(t / nx);
+  const int mi = i + oi, mj = j + oj, mk = k + ok;
+  if (mi < 0 || mi >= nx || mj < 0 || mj >= g.ny() || mk < 0 || mk >= nz) return;
+  const int64_t m = DIM == 3 ? ((int64_t)mj * nx + mi) * nz + mk : (int64_t)mj * nx + mi;
+  for (int r = 0; r < DOF; ++r)
+    for (int c = 0; c < DOF; ++c)
+      A[(n * DOF + r) * ld + m * DOF + c] = S[((int64_t)(o * DOF + r) * DOF + c) * N + n];
+}
+
+__global__ void pad_identity_kernel(int n, int ld, double* A) {
+  const int i = n + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ld) A[(int64_t)i * ld + i] = 1.0;
+}
+
+// ------------------------------------------------------ blocked Cholesky ---
+
+// Factor the diagonal tile (kb, kb) in place (lower triangle).  256 threads.
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int ld, int kb, int* info) {
+  __shared__ double T[NB][NB + 1];
+  const int base = kb * NB;
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    const int r = t / NB, c = t % NB;
+    T[r][c] = A[(int64_t)(base + r) * ld + base + c];
+  }
+  __syncthreads();
+  for (int j = 0; j < NB; ++j) {
+    if (threadIdx.x == 0) {
+      const double d = T[j][j];
+      if (!(d > 0.0)) *info = base + j + 1;
+      T[j][j] = sqrt(d > 0.0 ? d : 1.0);
+    }
+    __syncthreads();
+    const double piv = T[j][j];
+    for (int r = j + 1 + threadIdx.x; r < NB; r += blockDim.x) T[r][j] /= piv;
+    __syncthreads();
+    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+      const int r = t / NB, c = t % NB;
+      if (c > j && r >= c) T[r][c] -= T[r][j] * T[c][j];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    const int r = t / NB, c = t % NB;
+    A[(int64_t)(base + r) * ld + base + c] = r >= c ? T[r][c] : 0.0;
+  }
+}
+
+// Panel: A_ik <- A_ik L_kk^-T for tiles i > kb.  64 threads, one row each.
+__global__ __launch_bounds__(64) void trsm_panel_kernel(double* A, int ld, int kb) {
+  __shared__ double L[NB][NB + 1];
+  __shared__ double X[NB][NB + 1];
+  const int ib = kb + 1 + blockIdx.x;
+  const int base = kb * NB, rbase = ib * NB;
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    const int r = t / NB, c = t % NB;
+    L[r][c] = A[(int64_t)(base + r) * ld + base + c];
+    X[r][c] = A[(int64_t)(rbase + r) * ld + base + c];
+  }
+  __syncthreads();
+  const int r = threadIdx.x;
+  for (int j = 0; j < NB; ++j) {
+    double v = X[r][j];
+    for (int m = 0; m < j; ++m) v -= X[r][m] * L[j][m];
+    X[r][j] = v / L[j][j];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    const int rr = t / NB, c = t % NB;
+    A[(int64_t)(rbase + rr) * ld + base + c] = X[rr][c];
+  }
+}
+
+__device__ __forceinline__ void tri_index(int t, int& i, int& j) {
+  // t -> (i, j) with 0 <= j <= i
+  i = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((i + 1) * (i + 2) / 2 <= t) ++i;
+  while (i * (i + 1) / 2 > t) --i;
+  j = t - i * (i + 1) / 2;
+}
+
+// 64x64 tile product helper: C(4x4 per thread) += Atile * Btile^T, tiles in LDS.
+__device__ __forceinline__ void tile_mma_nt(const double (*Ta)[NB + 1], const double (*Tb)[NB + 1],
+                                            double (&acc)[4][4], int r0, int c0) {
+  for (int m = 0; m < NB; ++m) {
+    double a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = Ta[r0 + q][m];
+      b[q] = Tb[c0 + q][m];
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * b[q];
+  }
+}
+
+// Trailing update: A_ij -= L_ik L_jk^T for kb < j <= i.  256 threads per tile.
+__global__ __launch_bounds__(256) void syrk_update_kernel(double* A, int ld, int kb) {
+  __shared__ double Ta[NB][NB + 1];
+  __shared__ double Tb[NB][NB + 1];
+  int ti, tj;
+  tri_index(blockIdx.x, ti, tj);
+  const int ib = kb + 1 + ti, jb = kb + 1 + tj;
+  const int kbase = kb * NB;
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    const int r = t / NB, c = t % NB;
+    Ta[r][c] = A[(int64_t)(ib * NB + r) * ld + kbase + c];
+    Tb[r][c] = A[(int64_t)(jb * NB + r) * ld + kbase + c];
+  }
+  __syncthreads();
+  const int r0 = (threadIdx.x / 16) * 4, c0 = (threadIdx.x % 16) * 4;
+  double acc[4][4] = {};
+  tile_mma_nt(Ta, Tb, acc, r0, c0);
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) A[(int64_t)(ib * NB + r0 + p) * ld + jb * NB + c0 + q] -= acc[p][q];
+}
+
+// ------------------------------------------------------- inverse of L ---
+
+// Y = L^-1 by block rows: Y_ij = L_ii^-1 (delta_ij I - sum_{m=j}^{i-1} L_im Y_mj).
+// Launch per block row ib with ib+1 blocks (one per column tile jb <= ib).
+__global__ __launch_bounds__(256) void trtri_row_kernel(const double* L, double* Y, int ld, int ib) {
+  __shared__ double Ta[NB][NB + 1];
+  __shared__ double Tb[NB][NB + 1];
+  const int jb = blockIdx.x;
+  const int r0 = (threadIdx.x / 16) * 4, c0 = (threadIdx.x % 16) * 4;
+  double acc[4][4] = {};
+  for (int mb = jb; mb < ib; ++mb) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+      const int r = t / NB, c = t % NB;
+      Ta[r][c] = L[(int64_t)(ib * NB + r) * ld + mb * NB + c];
+      // store Y_mj transposed so tile_mma_nt computes L_im * Y_mj
+      Tb[c][r] = Y[(int64_t)(mb * NB + r) * ld + jb * NB + c];
+    }
+    __syncthreads();
+    tile_mma_nt(Ta, Tb, acc, r0, c0);
+  }
+  __syncthreads();
+  // B = delta I - acc, stored in Tb; L_ii in Ta
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    const int r = t / NB, c = t % NB;
+    Ta[r][c] = L[(int64_t)(ib * NB + r) * ld + ib * NB + c];
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      Tb[r0 + p][c0 + q] = ((ib == jb && r0 + p == c0 + q) ? 1.0 : 0.0) - acc[p][q];
+  __syncthreads();
+  // forward substitution per column: threads 0..63 own one column each
+  if (threadIdx.x < NB) {
+    const int c = threadIdx.x;
+    for (int r = 0; r < NB; ++r) {
+      double v = Tb[r][c];
+      for (int m = 0; m < r; ++m) v -= Ta[r][m] * Tb[m][c];
+      Tb[r][c] = v / Ta[r][r];
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    const int r = t / NB, c = t % NB;
+    Y[(int64_t)(ib * NB + r) * ld + jb * NB + c] = Tb[r][c];
+  }
+}
+
+// Ainv_ij = sum_{m >= i} Y_mi^T Y_mj for i >= j (tiles), mirrored to j > i.
+__global__ __launch_bounds__(256) void yty_kernel(const double* Y, double* Ainv, int ld, int nbt) {
+  __shared__ double Ta[NB][NB + 1];
+  __shared__ double Tb[NB][NB + 1];
+  int ib, jb;
+  tri_index(blockIdx.x, ib, jb);
+  const int r0 = (threadIdx.x / 16) * 4, c0 = (threadIdx.x % 16) * 4;
+  double acc[4][4] = {};
+  for (int mb = ib; mb < nbt; ++mb) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+      const int r = t / NB, c = t % NB;
+      Ta[c][r] = Y[(int64_t)(mb * NB + r) * ld + ib * NB + c];  // (Y_mi)^T
+      Tb[c][r] = Y[(int64_t)(mb * NB + r) * ld + jb * NB + c];  // (Y_mj)^T
+    }
+    __syncthreads();
+    tile_mma_nt(Ta, Tb, acc, r0, c0);
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int gi = ib * NB + r0 + p, gj = jb * NB + c0 + q;
+      Ainv[(int64_t)gi * ld + gj] = acc[p][q];
+      Ainv[(int64_t)gj * ld + gi] = acc[p][q];
+    }
+}
+
+// -------------------------------------------------------- coarse solve ---
+
+// x = Ainv b (first n entries); one wavefront per row.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void symv_kernel(const double* __restrict__ Ainv, int ld, int n,
+                                                   const TI* __restrict__ b, TO* __restrict__ x) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double s = 0.0;
+  const double* a = Ainv + (int64_t)row * ld;
+  for (int c = lane; c < n; c += 64) s += a[c] * (double)b[c];
+  s = wave_sum(s);
+  if (lane == 0) x[row] = (TO)s;
+}
+
+int CoarseSolver::init(int n_, hipStream_t s) {
+  n = n_;
+  ld = (n + NB - 1) / NB * NB;
+  nbt = ld / NB;
+  const size_t bytes = (size_t)ld * ld * sizeof(double);
+  ET_CHECK_HIP(hipMalloc(&A, bytes));
+  ET_CHECK_HIP(hipMalloc(&Y, bytes));
+  ET_CHECK_HIP(hipMalloc(&info, sizeof(int)));
+  ET_CHECK_HIP(hipHostMalloc(&info_h, sizeof(int)));
+  (void)s;
+  return ET_OK;
+}
+
+void CoarseSolver::release() {
+  if (A) (void)hipFree(A);
+  if (Y) (void)hipFree(Y);
+  if (info) (void)hipFree(info);
+  if (info_h) (void)hipHostFree(info_h);
+  A = Y = nullptr;
+  info = nullptr;
+  info_h = nullptr;
+}
+
+int CoarseSolver::factor(const Grid& g, int dof, const double* S, hipStream_t s) {
+  const size_t bytes = (size_t)ld * ld * sizeof(double);
+  ET_CHECK_HIP(hipMemsetAsync(A, 0, bytes, s));
+  ET_CHECK_HIP(hipMemsetAsync(Y, 0, bytes, s));
+  ET_CHECK_HIP(hipMemsetAsync(info, 0, sizeof(int), s));
+  const int NO = g.dim == 3 ? 27 : 9;
+  const unsigned grid = grid_for(g.n_nodes() * NO);
+  if (g.dim == 3)
+    dense_from_stencil_kernel<3, 3><<<grid, kBlock, 0, s>>>(g, S, ld, A);
+  else
+    dense_from_stencil_kernel<2, 2><<<grid, kBlock, 0, s>>>(g, S, ld, A);
+  ET_CHECK_LAUNCH();
+  if (ld > n) {
+    pad_identity_kernel<<<grid_for(ld - n), kBlock, 0, s>>>(n, ld, A);
+    ET_CHECK_LAUNCH();
+  }
+  for (int kb = 0; kb < nbt; ++kb) {
+    potrf_diag_kernel<<<1, 256, 0, s>>>(A, ld, kb, info);
+    ET_CHECK_LAUNCH();
+    const int rest = nbt - kb - 1;
+    if (rest > 0) {
+      trsm_panel_kernel<<<rest, 64, 0, s>>>(A, ld, kb);
+      ET_CHECK_LAUNCH();
+      syrk_update_kernel<<<rest * (rest + 1) / 2, 256, 0, s>>>(A, ld, kb);
+      ET_CHECK_LAUNCH();
+    }
+  }
+  // A now holds L in its lower triangle (upper part stale); Y = L^-1
+  for (int ib = 0; ib < nbt; ++ib) {
+    trtri_row_kernel<<<ib + 1, 256, 0, s>>>(A, Y, ld, ib);
+    ET_CHECK_LAUNCH();
+  }
+  // A <- Y^T Y = A_coarse^-1
+  yty_kernel<<<nbt * (nbt + 1) / 2, 256, 0, s>>>(Y, A, ld, nbt);
+  ET_CHECK_LAUNCH();
+  ET_CHECK_HIP(hipMemcpyAsync(info_h, info, sizeof(int), hipMemcpyDeviceToHost, s));
+  return ET_OK;
+}
+
+int CoarseSolver::check_info(hipStream_t s) {
+  ET_CHECK_HIP(hipStreamSynchronize(s));
+  if (*info_h != 0) {
+    set_error("coarse-grid Cholesky: matrix not positive definite at row " +
+              std::to_string(*info_h - 1));
+    return ET_ENONFINITE;
+  }
+  return ET_OK;
+}
+
+template <typename TI, typename TO>
+int CoarseSolver::solve(const TI* b, TO* x, hipStream_t s) const {
+  symv_kernel<TI, TO><<<(n + 3) / 4, 256, 0, s>>>(A, ld, n, b, x);
+  ET_CHECK_LAUNCH();
+  return ET_OK;
+}
+template int CoarseSolver::solve<double, double>(const double*, double*, hipStream_t) const;
+template int CoarseSolver::solve<float, float>(const float*, float*, hipStream_t) const;
+
+}  // namespace et

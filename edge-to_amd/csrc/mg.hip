@@ -1,0 +1,270 @@
+// Geometric multigrid building blocks on structured grids:
+//   * restriction (normalised full weighting) and prolongation,
+//   * Galerkin coarse operators S_c = R S_f R^T on dense block stencils,
+//   * block-stencil application.
+// Stencil layout: plane ((o*dof + r)*dof + c) holds the (r, c) entry of the
+// block coupling node n to neighbour n + offset(o), one value per node, so a
+// wavefront reads each plane with unit stride.
+#include "common.hpp"
+#include "stiffness.hpp"
+
+namespace et {
+
+__device__ __forceinline__ double pw(int d) {  // 2^-|d| for d in {-1, 0, 1}
+  return d == 0 ? 1.0 : 0.5;
+}
+
+template <int DIM>
+__device__ __forceinline__ void node_coords(const Grid& g, int64_t n, int& i, int& j, int& k) {
+  const int nz = g.nz(), nx = g.nx();
+  k = DIM == 3 ? (int)(n % nz) : 0;
+  const int64_t t = DIM == 3 ? n / nz : n;
+  i = (int)(t % nx);
+  j = (int)(t / nx);
+}
+
+template <int DIM>
+__device__ __forceinline__ int64_t node_index(const Grid& g, int i, int j, int k) {
+  return DIM == 3 ? ((int64_t)j * g.nx() + i) * g.nz() + k : (int64_t)j * g.nx() + i;
+}
+
+// total weight tw(c): product over axes of 1.5 on the boundary, 2 inside
+template <int DIM>
+__device__ __forceinline__ double tw_of(const Grid& gc, int i, int j, int k) {
+  double t = (i == 0 || i == gc.nx() - 1) ? 1.5 : 2.0;
+  t *= (j == 0 || j == gc.ny() - 1) ? 1.5 : 2.0;
+  if (DIM == 3) t *= (k == 0 || k == gc.nz() - 1) ? 1.5 : 2.0;
+  return t;
+}
+
+// ------------------------------------------------------------ restriction ---
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void restrict_kernel(const Grid gf, const Grid gc, int dof,
+                                                       const T* __restrict__ vf,
+                                                       T* __restrict__ vc) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= gc.n_nodes()) return;
+  int ci, cj, ck;
+  node_coords<DIM>(gc, c, ci, cj, ck);
+  const T tw = (T)tw_of<DIM>(gc, ci, cj, ck);
+  for (int d = 0; d < dof; ++d) {
+    T v = 0;
+    for (int di = -1; di <= 1; ++di) {
+      const int fi = 2 * ci + di;
+      if (fi < 0 || fi >= gf.nx()) continue;
+      for (int dj = -1; dj <= 1; ++dj) {
+        const int fj = 2 * cj + dj;
+        if (fj < 0 || fj >= gf.ny()) continue;
+        for (int dk = (DIM == 3 ? -1 : 0); dk <= (DIM == 3 ? 1 : 0); ++dk) {
+          const int fk = 2 * ck + dk;
+          if (DIM == 3 && (fk < 0 || fk >= gf.nz())) continue;
+          const T wgt = (T)(pw(di) * pw(dj) * pw(dk));
+          v += wgt * vf[node_index<DIM>(gf, fi, fj, fk) * dof + d];
+        }
+      }
+    }
+    vc[c * dof + d] = v / tw;
+  }
+}
+
+template <typename T>
+int restrict_op(const Grid& gf, int dof, const T* vf, T* vc, hipStream_t s) {
+  const Grid gc = gf.coarse();
+  const unsigned grid = grid_for(gc.n_nodes());
+  if (gf.dim == 3)
+    restrict_kernel<T, 3><<<grid, kBlock, 0, s>>>(gf, gc, dof, vf, vc);
+  else
+    restrict_kernel<T, 2><<<grid, kBlock, 0, s>>>(gf, gc, dof, vf, vc);
+  ET_CHECK_LAUNCH();
+  return ET_OK;
+}
+template int restrict_op<double>(const Grid&, int, const double*, double*, hipStream_t);
+template int restrict_op<float>(const Grid&, int, const float*, float*, hipStream_t);
+
+// ----------------------------------------------------------- prolongation ---
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void prolong_kernel(const Grid gf, const Grid gc, int dof,
+                                                      const T* __restrict__ vc,
+                                                      T* __restrict__ vf) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= gf.n_nodes()) return;
+  int i, j, k;
+  node_coords<DIM>(gf, f, i, j, k);
+  const int ci0 = i / 2, cj0 = j / 2, ck0 = k / 2;
+  for (int d = 0; d < dof; ++d) {
+    T v = 0;
+    int count = 0;
+    for (int ci = ci0; ci <= ci0 + (i % 2); ++ci) {
+      if (ci >= gc.nx()) continue;
+      for (int cj = cj0; cj <= cj0 + (j % 2); ++cj) {
+        if (cj >= gc.ny()) continue;
+        for (int ck = ck0; ck <= ck0 + (DIM == 3 ? (k % 2) : 0); ++ck) {
+          if (DIM == 3 && ck >= gc.nz()) continue;
+          const T tw = (T)tw_of<DIM>(gc, ci, cj, ck);
+          v += vc[node_index<DIM>(gc, ci, cj, ck) * dof + d] / tw;
+          ++count;
+        }
+      }
+    }
+    vf[f * dof + d] = count > 0 ? v / (T)count : T(0);
+  }
+}
+
+template <typename T>
+int prolong_op(const Grid& gf, int dof, const T* vc, T* vf, hipStream_t s) {
+  const Grid gc = gf.coarse();
+  const unsigned grid = grid_for(gf.n_nodes());
+  if (gf.dim == 3)
+    prolong_kernel<T, 3><<<grid, kBlock, 0, s>>>(gf, gc, dof, vc, vf);
+  else
+    prolong_kernel<T, 2><<<grid, kBlock, 0, s>>>(gf, gc, dof, vc, vf);
+  ET_CHECK_LAUNCH();
+  return ET_OK;
+}
+template int prolong_op<double>(const Grid&, int, const double*, double*, hipStream_t);
+template int prolong_op<float>(const Grid&, int, const float*, float*, hipStream_t);
+
+// --------------------------------------------------------------- Galerkin ---
+
+// One thread per (coarse node c, neighbour offset o): the dof x dof block of
+// R S_f R^T coupling c and t = c + o.
+template <int DIM, int DOF>
+__global__ __launch_bounds__(256) void galerkin_kernel(const Grid gf, const Grid gc,
+                                                       const double* __restrict__ Sf,
+                                                       double* __restrict__ Sc) {
+  constexpr int NO = DIM == 3 ? 27 : 9;
+  const int64_t Nc = gc.n_nodes(), Nf = gf.n_nodes();
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= Nc * NO) return;
+  const int o = (int)(tid / Nc);
+  const int64_t c = tid % Nc;
+  const int oi = DIM == 3 ? o / 9 - 1 : o / 3 - 1;
+  const int oj = DIM == 3 ? (o / 3) % 3 - 1 : o % 3 - 1;
+  const int ok = DIM == 3 ? o % 3 - 1 : 0;
+  int ci, cj, ck;
+  node_coords<DIM>(gc, c, ci, cj, ck);
+  const int ti = ci + oi, tj = cj + oj, tk = ck + ok;
+  double blk[DOF][DOF];
+#pragma unroll
+  for (int r = 0; r < DOF; ++r)
+#pragma unroll
+    for (int q = 0; q < DOF; ++q) blk[r][q] = 0.0;
+  const bool tin = ti >= 0 && ti < gc.nx() && tj >= 0 && tj < gc.ny() &&
+                   (DIM == 2 || (tk >= 0 && tk < gc.nz()));
+  if (tin) {
+    for (int di = -1; di <= 1; ++di) {
+      const int fi = 2 * ci + di;
+      if (fi < 0 || fi >= gf.nx()) continue;
+      for (int dj = -1; dj <= 1; ++dj) {
+        const int fj = 2 * cj + dj;
+        if (fj < 0 || fj >= gf.ny()) continue;
+        for (int dk = (DIM == 3 ? -1 : 0); dk <= (DIM == 3 ? 1 : 0); ++dk) {
+          const int fk = 2 * ck + dk;
+          if (DIM == 3 && (fk < 0 || fk >= gf.nz())) continue;
+          const double wf = pw(di) * pw(dj) * pw(dk);
+          const int64_t f = node_index<DIM>(gf, fi, fj, fk);
+          // fine neighbours g = 2t + dg that are stencil neighbours of f
+          for (int gi = -1; gi <= 1; ++gi) {
+            const int si = 2 * ti + gi - fi;
+            if (si < -1 || si > 1 || 2 * ti + gi < 0 || 2 * ti + gi >= gf.nx()) continue;
+            for (int gj = -1; gj <= 1; ++gj) {
+              const int sj = 2 * tj + gj - fj;
+              if (sj < -1 || sj > 1 || 2 * tj + gj < 0 || 2 * tj + gj >= gf.ny()) continue;
+              for (int gk = (DIM == 3 ? -1 : 0); gk <= (DIM == 3 ? 1 : 0); ++gk) {
+                const int sk = DIM == 3 ? 2 * tk + gk - fk : 0;
+                if (DIM == 3 && (sk < -1 || sk > 1 || 2 * tk + gk < 0 || 2 * tk + gk >= gf.nz()))
+                  continue;
+                const double wgt = wf * pw(gi) * pw(gj) * pw(gk);
+                const int so = DIM == 3 ? (si + 1) * 9 + (sj + 1) * 3 + (sk + 1)
+                                        : (si + 1) * 3 + (sj + 1);
+#pragma unroll
+                for (int r = 0; r < DOF; ++r)
+#pragma unroll
+                  for (int q = 0; q < DOF; ++q)
+                    blk[r][q] += wgt * Sf[((int64_t)(so * DOF + r) * DOF + q) * Nf + f];
+              }
+            }
+          }
+        }
+      }
+    }
+    const double inv = 1.0 / (tw_of<DIM>(gc, ci, cj, ck) * tw_of<DIM>(gc, ti, tj, tk));
+#pragma unroll
+    for (int r = 0; r < DOF; ++r)
+#pragma unroll
+      for (int q = 0; q < DOF; ++q) blk[r][q] *= inv;
+  }
+#pragma unroll
+  for (int r = 0; r < DOF; ++r)
+#pragma unroll
+    for (int q = 0; q < DOF; ++q) Sc[((int64_t)(o * DOF + r) * DOF + q) * Nc + c] = blk[r][q];
+}
+
+int galerkin(const Grid& gf, int dof, const double* Sf, double* Sc, hipStream_t s) {
+  const Grid gc = gf.coarse();
+  const int NO = gf.dim == 3 ? 27 : 9;
+  const unsigned grid = grid_for(gc.n_nodes() * NO);
+  if (gf.dim == 3) {
+    ET_REQUIRE(dof == 3, "3-D Galerkin expects dof 3");
+    galerkin_kernel<3, 3><<<grid, kBlock, 0, s>>>(gf, gc, Sf, Sc);
+  } else {
+    ET_REQUIRE(dof == 2, "2-D Galerkin expects dof 2");
+    galerkin_kernel<2, 2><<<grid, kBlock, 0, s>>>(gf, gc, Sf, Sc);
+  }
+  ET_CHECK_LAUNCH();
+  return ET_OK;
+}
+
+// ------------------------------------------------------- stencil apply ---
+
+template <typename T, int DIM, int DOF>
+__global__ __launch_bounds__(256) void stencil_apply_kernel(const Grid g, const T* __restrict__ S,
+                                                            const T* __restrict__ x,
+                                                            T* __restrict__ y) {
+  constexpr int NO = DIM == 3 ? 27 : 9;
+  const int64_t N = g.n_nodes();
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  int i, j, k;
+  node_coords<DIM>(g, n, i, j, k);
+  T acc[DOF];
+#pragma unroll
+  for (int r = 0; r < DOF; ++r) acc[r] = 0;
+#pragma unroll
+  for (int o = 0; o < NO; ++o) {
+    const int oi = DIM == 3 ? o / 9 - 1 : o / 3 - 1;
+    const int oj = DIM == 3 ? (o / 3) % 3 - 1 : o % 3 - 1;
+    const int ok = DIM == 3 ? o % 3 - 1 : 0;
+    const int mi = i + oi, mj = j + oj, mk = k + ok;
+    if (mi < 0 || mi >= g.nx() || mj < 0 || mj >= g.ny()) continue;
+    if (DIM == 3 && (mk < 0 || mk >= g.nz())) continue;
+    const int64_t m = node_index<DIM>(g, mi, mj, mk);
+#pragma unroll
+    for (int q = 0; q < DOF; ++q) {
+      const T xv = x[m * DOF + q];
+#pragma unroll
+      for (int r = 0; r < DOF; ++r) acc[r] += S[((int64_t)(o * DOF + r) * DOF + q) * N + n] * xv;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < DOF; ++r) y[n * DOF + r] = acc[r];
+}
+
+template <typename T>
+int stencil_apply(const Grid& g, int dof, const T* S, const T* x, T* y, hipStream_t s) {
+  const unsigned grid = grid_for(g.n_nodes());
+  if (g.dim == 3)
+    stencil_apply_kernel<T, 3, 3><<<grid, kBlock, 0, s>>>(g, S, x, y);
+  else
+    stencil_apply_kernel<T, 2, 2><<<grid, kBlock, 0, s>>>(g, S, x, y);
+  ET_CHECK_LAUNCH();
+  return ET_OK;
+}
+template int stencil_apply<double>(const Grid&, int, const double*, const double*, double*,
+                                   hipStream_t);
+template int stencil_apply<float>(const Grid&, int, const float*, const float*, float*,
+                                  hipStream_t);
+
+}  // namespace et

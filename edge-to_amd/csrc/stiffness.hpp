@@ -1,0 +1,50 @@
+// Internal declarations shared between the kernel translation units, the
+// solver and the C-ABI layer.
+#pragma once
+#include "common.hpp"
+
+namespace et {
+
+int generate_elements(const Grid& g, int32_t* el, hipStream_t s);
+
+template <typename T>
+int matvec(const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T* u, T* y,
+           hipStream_t s);
+template <typename T>
+int diagonal(const Grid& g, const T* K0, const T* E, const uint8_t* cons, T* diag, hipStream_t s);
+
+// mode 0: -u^T K0 u; mode 1: SIMP sensitivity p*clip(xt)^(p-1) * (-u^T K0 u)
+int elem_energy(const Grid& g, const double* K0, const double* U, const double* xt, double penal,
+                double vmin, int mode, double* out, hipStream_t s);
+int simp_forward(int64_t n, const double* xt, double penal, double vmin, int plain, double* E,
+                 hipStream_t s);
+int stencil_from_operator(const Grid& g, const double* K0, const double* E, const uint8_t* cons,
+                          double* S, hipStream_t s);
+
+// filter
+int filter_normalizer(const Grid& g, int n_off, const int32_t* off, const double* w, double* norm,
+                      hipStream_t s);
+int filter_fwd(const Grid& g, int n_off, const int32_t* off, const double* w, const double* x,
+               double* y, hipStream_t s);
+int filter_adj(const Grid& g, int n_off, const int32_t* off, const double* w, const double* norm,
+               const double* v, double* y, hipStream_t s);
+
+// transfers (g = fine grid)
+template <typename T>
+int restrict_op(const Grid& g, int dof, const T* vf, T* vc, hipStream_t s);
+template <typename T>
+int prolong_op(const Grid& g, int dof, const T* vc, T* vf, hipStream_t s);
+
+// stencils (double)
+int galerkin(const Grid& g, int dof, const double* Sf, double* Sc, hipStream_t s);
+template <typename T>
+int stencil_apply(const Grid& g, int dof, const T* S, const T* x, T* y, hipStream_t s);
+
+// optimality criteria
+int64_t oc_work_size(int64_t n);
+int oc_update(int64_t n, const double* x, const double* dc, double A, double vol, double vf,
+              double move, double* xn, void* work, int* steps, hipStream_t s);
+int sqdist(int64_t n, const double* a, const double* b, double* partial, double* out,
+           hipStream_t s);
+
+}  // namespace et

@@ -1,0 +1,199 @@
+"""SIMP minimum-compliance optimiser (device).
+
+Mirrors EDGETO/TopOpt/_cuda.py (TopOpt / CuTopOpt, :17-382): constructor
+arguments, BC and load setup (add_BC_nodal / add_F_nodal :200-240, add_BCs /
+add_Forces :144-198 via a host KD-tree), update_design (:242-280),
+solve_and_process (:282-302) and optimize (:304-382).  Deviations, on
+purpose: rho_init works (the reference reads rho before assignment at
+:321-322; the intended clip(x0, void, 1) is used), and the OC bisection runs on
+the device with one host synchronisation per chunk of steps.
+"""
+import logging
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+
+logger = logging.getLogger(__name__)
+
+
+class TopOpt:
+    def __init__(self, mesh, material_model, kernel, solver, filter_kernel=None, max_iter=500,
+                 move=0.2, ch_tol=1e-4, fun_tol=1e-6, reuse_sol=True, solve_min_tol=np.inf,
+                 min_tol_patience=10, abandon_patience=10, verbose=True):
+        self.mesh = mesh
+        self.material_model = material_model
+        self.kernel = kernel
+        self.solver = solver
+        self.filter_kernel = filter_kernel
+        self.max_iter = max_iter
+        self.move = move
+        self.ch_tol = ch_tol
+        self.fun_tol = fun_tol
+        self.solve_min_tol = solve_min_tol
+        self.min_tol_patience = min_tol_patience
+        self.abandon_patience = abandon_patience
+        self.reuse_sol = reuse_sol
+        self.verbose = verbose
+        self.dtype = mesh.dtype
+        if verbose:
+            logger.setLevel(logging.INFO)
+        dev = kernel.constraints.device
+        self.f = torch.zeros(kernel.shape[0], dtype=torch.float64, device=dev)
+        self.KDTree = None
+        n = kernel.elements.shape[0]
+        self._oc_work = torch.empty(int(_lib.lib().et_oc_work_size(n)) // 8 + 1, dtype=torch.float64,
+                                    device=dev)
+        self._scalar = torch.zeros(4, dtype=torch.float64, device=dev)
+        self.iteration_log = []
+
+    # ------------------------------------------------------------- BCs / loads
+    def reset_BC(self):
+        self.kernel.set_constraints([])
+        self.kernel.has_con = False
+
+    def reset_F(self):
+        self.f[:] = 0
+
+    def _tree(self):
+        if self.KDTree is None:
+            from scipy.spatial import KDTree
+            self.KDTree = KDTree(self.mesh.nodes.cpu().numpy())
+        return self.KDTree
+
+    def add_BCs(self, positions, BCs):
+        _, idx = self._tree().query(np.asarray(positions))
+        BCs = np.asarray(BCs)
+        if BCs.shape[0] != idx.shape[0] and BCs.shape[0] > 1:
+            raise ValueError("BCs shape does not match positions shape and is not broadcastable.")
+        self.add_BC_nodal(np.asarray(idx), BCs)
+
+    def add_Forces(self, positions, Fs):
+        _, idx = self._tree().query(np.asarray(positions))
+        Fs = np.asarray(Fs, dtype=np.float64)
+        if Fs.shape[0] != idx.shape[0] and Fs.shape[0] > 1:
+            raise ValueError("Fs shape does not match positions shape and is not broadcastable.")
+        idx = np.asarray(idx).reshape(-1)
+        dof = self.mesh.dof
+        for i in range(dof):
+            vals = Fs[0, i] if Fs.shape[0] == 1 else Fs[:, i]
+            self.f[torch.as_tensor(idx * dof + i, device=self.f.device)] = torch.as_tensor(
+                vals, dtype=torch.float64, device=self.f.device)
+
+    def add_BC_nodal(self, node_ids, BCs):
+        node_ids = np.asarray(node_ids).reshape(-1)
+        BCs = np.asarray(BCs).astype(bool)
+        if BCs.shape[0] != node_ids.shape[0] and BCs.shape[0] > 1:
+            raise ValueError("BCs shape does not match positions shape and is not broadcastable.")
+        dof = self.mesh.dof
+        for i in range(dof):
+            if BCs.shape[0] == 1:
+                if BCs[0, i]:
+                    self.kernel.add_constraints(node_ids * dof + i)
+            else:
+                self.kernel.add_constraints(node_ids[BCs[:, i]] * dof + i)
+
+    def add_F_nodal(self, node_ids, Fs):
+        node_ids = np.asarray(node_ids)
+        Fs = np.asarray(Fs, dtype=np.float64)
+        if Fs.shape[0] != node_ids.shape[0] and Fs.shape[0] > 1:
+            raise ValueError("Fs shape does not match positions shape and is not broadcastable.")
+        dof = self.mesh.dof
+        ids = node_ids.reshape(-1)
+        for i in range(dof):
+            vals = Fs[0, i] if Fs.shape[0] == 1 else Fs[:, i]
+            self.f[torch.as_tensor(ids * dof + i, device=self.f.device)] = torch.as_tensor(
+                vals, dtype=torch.float64, device=self.f.device)
+
+    # --------------------------------------------------------------- OC / loop
+    def update_design(self, rho, df):
+        rho = rho.reshape(-1).contiguous()
+        df = df.reshape(-1).contiguous()
+        out = torch.empty_like(rho)
+        steps = ctypes_int()
+        _lib.call("et_oc_update_f64", rho.numel(), _lib.ptr(rho), _lib.ptr(df), float(self.mesh.A_host),
+                  float(self.mesh.volume), float(self.material_model.volume_fraction), float(self.move),
+                  _lib.ptr(out), _lib.ptr(self._oc_work), steps, _lib.stream())
+        self.last_oc_steps = steps.value
+        return out
+
+    def _dot(self, a, b):
+        _lib.call("et_dot_f64", a.numel(), _lib.ptr(a), _lib.ptr(b), _lib.ptr(self._scalar[0:1]),
+                  _lib.stream())
+        return float(self._scalar[0].item())
+
+    def _rms_change(self, a, b):
+        _lib.call("et_sqdist_f64", a.numel(), _lib.ptr(a), _lib.ptr(b), _lib.ptr(self._scalar[1:2]),
+                  _lib.stream())
+        return float(np.sqrt(self._scalar[1].item() / a.numel()))
+
+    def solve_and_process(self, rho, iteration):
+        mm = self.material_model
+        rho_ = self.filter_kernel.dot(rho)
+        rho__ = mm(rho_, iteration)
+        self.kernel.set_rho(rho__)
+        U, residual = self.solver.solve(self.f, use_last=self.reuse_sol)
+        compliance = self._dot(self.f, U)
+        pen = mm.penalty_schedule(mm.penalty, iteration) if mm.penalty_schedule else mm.penalty
+        dr = self.kernel.sensitivity(U, rho_, pen, mm.void)
+        dr = self.filter_kernel._rmatvec(dr)
+        return compliance, dr.reshape(-1, 1), residual
+
+    def optimize(self, save_comp_history=False, save_change_history=False, save_rho_history=False,
+                 rho_init=None):
+        flag = False
+        if rho_init is None:
+            rho = self.material_model.init_desvars(self.kernel.elements.shape[0])
+        else:
+            x0 = torch.as_tensor(rho_init, dtype=torch.float64, device=self.f.device).reshape(-1)
+            rho = self.material_model(x0.contiguous(), 0, plain=True)
+        self.kernel.set_rho(rho)
+        t0 = time.perf_counter()
+        comp, df, res = self.solve_and_process(rho, 0)
+        self.iteration_log = [dict(it=-1, comp=comp, res=res, pcg=self.solver.last_iterations,
+                                   t=time.perf_counter() - t0, setup_ms=self.solver.last_timings[0],
+                                   pcg_ms=self.solver.last_timings[1], oc_steps=0)]
+        comp_history, change_history, rho_history = [], [], []
+        for i in range(self.max_iter):
+            t0 = time.perf_counter()
+            rho_old = rho.clone()
+            comp_old = comp
+            rho = self.update_design(rho, df)
+            comp, df, res = self.solve_and_process(rho, i)
+            change = self._rms_change(rho, rho_old)
+            change_f = abs(comp - comp_old) / comp
+            self.iteration_log.append(dict(it=i, comp=comp, res=res, pcg=self.solver.last_iterations,
+                                           t=time.perf_counter() - t0,
+                                           setup_ms=self.solver.last_timings[0],
+                                           pcg_ms=self.solver.last_timings[1],
+                                           oc_steps=self.last_oc_steps, change=change))
+            if save_change_history:
+                change_history.append(change)
+            if save_rho_history:
+                rho_history.append(rho.cpu().numpy())
+            if save_comp_history:
+                comp_history.append(comp)
+            if self.verbose:
+                logger.info(f"it {i}: Compliance: {comp:.4e}, Change: {change:.4e}, "
+                            f"Function Change: {change_f:.4e}, Residual: {res:.4e}")
+            if change < self.ch_tol and change_f < self.fun_tol and self.material_model.is_terminal(i):
+                flag = True
+                break
+        hist = {}
+        if save_comp_history:
+            hist["comp_history"] = comp_history
+        if save_change_history:
+            hist["change_history"] = change_history
+        if save_rho_history:
+            hist["rho_history"] = rho_history
+        return rho, flag, hist
+
+
+def ctypes_int():
+    import ctypes
+    return ctypes.c_int(0)
+
+
+CuTopOpt = TopOpt

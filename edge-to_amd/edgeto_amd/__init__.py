@@ -1,0 +1,36 @@
+"""edgeto_amd -- MI355X-native drop-in for EDGE-TO's structured SIMP hot path.
+
+The public names are those of the reference package (EDGETO/__init__.py),
+restricted to the structured GPU path:
+
+    CuStructuredMesh2D / CuStructuredMesh3D     geom
+    CuStructuredFilter2D / CuStructuredFilter3D geom
+    CuStructuredStiffnessKernel                 kernels
+    CuMultiGrid                                 solvers
+    SingleMaterial                              MaterialModels
+    CuTopOpt                                    TopOpt
+    auto_stiffness                              physics
+
+plus compliance(), a torch.autograd.Function with a hand-written adjoint.
+All numerical work runs in libedgeto_amd.so (HIP, gfx950); importing this
+package loads it and raises if it is missing.
+"""
+from . import _lib
+from .geom import (CuStructuredFilter2D, CuStructuredFilter3D, CuStructuredMesh2D,
+                   CuStructuredMesh3D, FilterKernel, StructuredFilter2D, StructuredFilter3D,
+                   StructuredMesh2D, StructuredMesh3D)
+from .kernels import CuStructuredStiffnessKernel, StiffnessKernel, StructuredStiffnessKernel
+from .MaterialModels import MaterialModel, SingleMaterial
+from .physics import (auto_stiffness, hexahedron_element_stiffness,
+                      quadrilateral_element_stiffness)
+from .solvers import CuMultiGrid, MultiGrid, Solver
+from .TopOpt import CuTopOpt, TopOpt
+
+_lib.load(require_gpu=False)
+
+__all__ = ["CuStructuredMesh2D", "CuStructuredMesh3D", "CuStructuredFilter2D",
+           "CuStructuredFilter3D", "CuStructuredStiffnessKernel", "CuMultiGrid", "SingleMaterial",
+           "CuTopOpt", "auto_stiffness", "StructuredMesh2D", "StructuredMesh3D",
+           "StructuredFilter2D", "StructuredFilter3D", "StructuredStiffnessKernel", "MultiGrid",
+           "TopOpt", "FilterKernel", "StiffnessKernel", "MaterialModel", "Solver",
+           "quadrilateral_element_stiffness", "hexahedron_element_stiffness"]

@@ -1,0 +1,186 @@
+/*
+ * edgeto_amd.h -- C ABI of the MI355X-native EDGE-TO SIMP hot path.
+ *
+ * Every pointer argument named as device memory is a HIP device pointer owned
+ * by the caller; the library never frees caller memory.  It allocates
+ * device memory only in et_mg_create (solver workspace) and, once per host
+ * thread, a small reduction scratch on the first et_dot_f64 / et_sqdist_f64 /
+ * et_oc_update_f64 call.  Every entry point returns an
+ * int status (ET_OK = 0); on failure et_last_error() returns a message for the
+ * calling thread.  Work is enqueued on the given stream and is asynchronous
+ * unless stated otherwise.  No C++ exceptions cross this boundary.
+ *
+ * Indexing follows the reference exactly (bit-exact target):
+ *   nodes     2-D j*nx + i,              3-D j*nx*nz + i*nz + k   (nx = nelx+1, ...)
+ *   elements  2-D i*nely + j,            3-D i*nely*nelz + j*nelz + k
+ *   DOFs      node*dof + d, interleaved
+ * `nel` is the element count per axis (x, y[, z]); `dim` is 2 (Q4, dof 2) or
+ * 3 (H8, dof 3).  K0 is the row-major element matrix (8x8 or 24x24).
+ * Constraint masks are one byte per DOF (the reference's bool array).
+ *
+ * Reference citations are paths inside the EDGE-TO repository.
+ */
+#ifndef EDGETO_AMD_H
+#define EDGETO_AMD_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ET_OK = 0,
+  ET_EINVAL = 1,      /* bad argument (shape, dtype, null pointer, size limit) */
+  ET_EHIP = 2,        /* HIP runtime error */
+  ET_ENONFINITE = 3,  /* non-finite value detected */
+  ET_EBREAKDOWN = 4,  /* PCG breakdown: p^T A p <= 0 */
+  ET_ENOCONV = 5,     /* maxiter reached without meeting the tolerance */
+  ET_ENOMEM = 6       /* device allocation failed */
+};
+
+/* Message of the last failing call on this thread (empty string if none). */
+const char* et_last_error(void);
+
+/* Library version string. */
+const char* et_version(void);
+
+/* ---------------------------------------------------------------- mesh --- */
+
+/* Element connectivity (nel x 4|8, int32), bit-exact to
+ * generate_elements_2d/3d (core/_cuda.py:1089-1135, core/_geom.py:5-59). */
+int et_generate_elements(int dim, const int* nel, int32_t* elements, hipStream_t stream);
+
+/* ------------------------------------------------ stiffness operator --- */
+
+/* y = A_hat u with A_hat = M K(E) M + I_c: constrained rows copy u, constrained
+ * columns are skipped.  Replaces mat_vec_node_basis_parallel_wcon_cuda_kernel
+ * (core/_templated_cuda.cu:782-834) behind StructuredStiffnessKernel._matvec
+ * (kernels/_cuda.py:173-204).  cons may be NULL (no constraints, :137-178). */
+int et_matvec_f64(int dim, const int* nel, const double* K0, const double* E,
+                  const uint8_t* cons, const double* u, double* y, hipStream_t stream);
+int et_matvec_f32(int dim, const int* nel, const float* K0, const float* E,
+                  const uint8_t* cons, const float* u, float* y, hipStream_t stream);
+
+/* diag(A_hat); constrained entries are 1.  Replaces
+ * get_diagonal_node_basis_cuda_kernel (core/_templated_cuda.cu:1068-1113). */
+int et_diagonal_f64(int dim, const int* nel, const double* K0, const double* E,
+                    const uint8_t* cons, double* diag, hipStream_t stream);
+
+/* dk_e = -u_e^T K0 u_e on the unmasked U.  Replaces process_dk_kernel_cuda
+ * (core/_templated_cuda.cu:180-194) behind process_grad (kernels/_cuda.py:152-156). */
+int et_elem_energy_f64(int dim, const int* nel, const double* K0, const double* U,
+                       double* dk, hipStream_t stream);
+
+/* Fused SIMP sensitivity before the filter transpose:
+ * out_e = p * clip(xt_e, void, 1)^(p-1) * (-u_e^T K0 u_e).  Replaces the pair
+ * process_dk + SingleMaterial.__backward__ (MaterialModels.py:62-73) used in
+ * CuTopOpt.solve_and_process (TopOpt/_cuda.py:293-295). */
+int et_sensitivity_f64(int dim, const int* nel, const double* K0, const double* U,
+                       const double* xt, double penal, double void_, double* out,
+                       hipStream_t stream);
+
+/* E = clip(xt^p, void, 1) (plain: clip(xt, void, 1)), MaterialModels.py:47-60. */
+int et_simp_forward_f64(int64_t n, const double* xt, double penal, double void_, int plain,
+                        double* E, hipStream_t stream);
+
+/* ------------------------------------------------------ density filter --- */
+/* offsets: device int32 (n_off x dim); w: device weights (n_off).
+ * Normaliser N_i = sum of in-bounds weights: get_filter_{2D,3D}_weights_kernel
+ * (core/_templated_cuda.cu:1585-1630). */
+int et_filter_normalizer_f64(int dim, const int* nel, int n_off, const int32_t* offsets,
+                             const double* w, double* norm, hipStream_t stream);
+/* y = H x: apply_filter_{2D,3D}_kernel (core/_templated_cuda.cu:1526-1583). */
+int et_filter_fwd_f64(int dim, const int* nel, int n_off, const int32_t* offsets,
+                      const double* w, const double* x, double* y, hipStream_t stream);
+/* y = H^T v: apply_filter_{2D,3D}_transpose_kernel (core/_templated_cuda.cu:1632-1679). */
+int et_filter_adj_f64(int dim, const int* nel, int n_off, const int32_t* offsets,
+                      const double* w, const double* norm, const double* v, double* y,
+                      hipStream_t stream);
+
+/* ------------------------------------------------- multigrid transfers --- */
+/* Restriction fine -> coarse (nel = fine element counts): restriction_{2d,3d}_kernel
+ * (core/_templated_cuda.cu:232-364) behind apply_restriction_cuda (core/_mgm_cuda.py:223-268). */
+int et_restrict_f64(int dim, const int* nel, int dof, const double* vf, double* vc,
+                    hipStream_t stream);
+/* Prolongation coarse -> fine: prolongation_{2d,3d}_kernel (core/_templated_cuda.cu:366-475)
+ * behind apply_prolongation_cuda (core/_mgm_cuda.py:270-314). */
+int et_prolong_f64(int dim, const int* nel, int dof, const double* vc, double* vf,
+                   hipStream_t stream);
+
+/* Galerkin coarse operators as dense block stencils.  Layout: for the 3^dim
+ * neighbour offsets o (o = (di+1)*9 + (dj+1)*3 + (dk+1) in 3-D, (di+1)*3 + (dj+1)
+ * in 2-D) and dof x dof block entries (r, c), plane (o*dof + r)*dof + c holds one
+ * value per node: S[((o*dof + r)*dof + c) * n_nodes + node].
+ * et_stencil_from_operator: stencil of A_hat on the fine grid (nel).
+ * et_galerkin: S_coarse = R S_fine R^T (nel = fine element counts), replacing
+ * get_restricted_{2d,3d}_{l0,l1p}_nnz_based (core/_mgm_cuda.cu:95-344, 430-649). */
+int et_stencil_from_operator_f64(int dim, const int* nel, const double* K0, const double* E,
+                                 const uint8_t* cons, double* S, hipStream_t stream);
+int et_galerkin_f64(int dim, const int* nel, int dof, const double* S_fine, double* S_coarse,
+                    hipStream_t stream);
+/* y = S x for a stencil operator on grid nel. */
+int et_stencil_apply_f64(int dim, const int* nel, int dof, const double* S, const double* x,
+                         double* y, hipStream_t stream);
+
+/* ------------------------------------------------------- vector algebra --- */
+/* *out = a . b (device scalar), deterministic two-stage reduction.
+ * Replaces the cuBLAS dot / nrm2 calls of CuMultiGrid.solve (solvers/_multigrid_cuda.py:214-231). */
+int et_dot_f64(int64_t n, const double* a, const double* b, double* out, hipStream_t stream);
+
+/* *out = sum (a - b)^2 (device scalar); the RMS design change of the SIMP loop
+ * (TopOpt/_cuda.py:350) is sqrt(*out / n). */
+int et_sqdist_f64(int64_t n, const double* a, const double* b, double* out, hipStream_t stream);
+
+/* ------------------------------------------------- optimality criteria --- */
+/* One OC design update (TopOpt/_cuda.py:242-280 with SingleMaterial.__ocP__ /
+ * __con__, MaterialModels.py:75-90): bisection on lambda in [1e-9, 1e9] until
+ * (l2-l1)/(l2+l1) <= 1e-4; the returned design is the one at the last midpoint.
+ * A = element measure (structured: one value), vol = A * n, vf = volume fraction.
+ * Synchronises `stream` (the bisection count is data-dependent); *steps receives
+ * the number of bisection steps.  work: device buffer of et_oc_work_size(n) bytes. */
+int64_t et_oc_work_size(int64_t n);
+int et_oc_update_f64(int64_t n, const double* x, const double* dc, double A, double vol,
+                     double vf, double move, double* x_new, void* work, int* steps,
+                     hipStream_t stream);
+
+/* ------------------------------------------------------- GMG-PCG solver --- */
+/* Opaque solver: the CuMultiGrid object (solvers/_multigrid_cuda.py:15-242) with
+ * its hierarchy, workspace and coarse factor.  precision: 0 = FP64 throughout;
+ * 1 = mixed (FP32 V-cycle, FP64 outer PCG, coarse solve and residual).
+ * cycle_w != 0 selects the 'w' branch for levels whose bit is set in w_level_mask
+ * (the reference only takes it for a lowercase 'w', :165). */
+typedef struct et_mg et_mg;
+
+typedef struct et_mg_params {
+  int dim;
+  int nel[3];
+  int n_level;       /* number of coarsenings (grids = n_level + 1) */
+  int n_smooth;      /* damped-Jacobi sweeps before and after the coarse correction */
+  double omega;      /* reference `omega` (max omega) */
+  double min_omega;  /* reference `min_omega`; min = max(omega/2, min_omega) */
+  int cycle_w;
+  int w_level_mask;
+  int precision;
+} et_mg_params;
+
+int et_mg_create(et_mg** handle, const et_mg_params* params, const double* K0_host,
+                 hipStream_t stream);
+int et_mg_destroy(et_mg* handle);
+
+/* Solve A_hat(E) x = b by GMG-preconditioned CG (solve(), :188-242).  E, cons,
+ * b, x are device pointers (E: n_elem, cons: n_dof bytes, b, x: n_dof).  x holds
+ * the initial guess on entry (the caller zeroes it when use_last is false) and
+ * the solution on exit.  The hierarchy is rebuilt from E on every call, as the
+ * reference does.  Synchronises `stream`.  Outputs: iterations, |r|/|b|. */
+int et_mg_solve(et_mg* handle, const double* E, const uint8_t* cons, const double* b,
+                double* x, double tol, int maxiter, int* iters, double* residual);
+
+/* Per-phase timings of the last solve in milliseconds (setup, pcg), via HIP events. */
+int et_mg_last_timings(et_mg* handle, double* setup_ms, double* pcg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EDGETO_AMD_H */

@@ -1,0 +1,263 @@
+"""GPU parity of every native kernel against the CPU oracle, through the C ABI.
+
+Tolerances: integer and index work bit-exact; FP64 operators 1e-12 relative
+(max-norm, relative to the max-norm of the reference output).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import filters as ofilt
+from oracle import mesh as omesh
+from oracle import mgm as omgm
+from oracle import ops as oops
+from oracle import physics as ophys
+from oracle import topopt as otop
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+MESHES = [(7, 5), (12, 8), (5, 4, 3), (8, 6, 4), (6, 4, 4)]
+
+
+def _lib():
+    from edgeto_amd import _lib as L
+    L.load()
+    return L
+
+
+_KEEP = []
+
+
+@pytest.fixture(autouse=True)
+def _release_inputs():
+    """Device inputs stay referenced until the test ends: a pointer handed to the
+    C ABI must outlive the call (torch's allocator reuses freed blocks at once)."""
+    yield
+    torch.cuda.synchronize()
+    _KEEP.clear()
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    t = t.cuda()
+    _KEEP.append(t)
+    return t
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _problem(nel, seed=0, nu=0.3, frac_cons=0.15):
+    rng = np.random.default_rng(seed)
+    dim = len(nel)
+    el = omesh.elements_3d(*nel) if dim == 3 else omesh.elements_2d(*nel)
+    nodes = omesh.nodes_3d(*nel, *nel) if dim == 3 else omesh.nodes_2d(*nel, *nel)
+    K0, _, _, _ = ophys.auto_stiffness(nodes[el[0]], 1.0, nu)
+    n_dof = nodes.shape[0] * dim
+    E = rng.uniform(1e-3, 1.0, el.shape[0])
+    u = rng.standard_normal(n_dof)
+    cons = rng.random(n_dof) < frac_cons
+    return dict(nel=nel, dim=dim, el=el, nodes=nodes, K0=K0, E=E, u=u, cons=cons, n_dof=n_dof)
+
+
+@pytest.mark.parametrize("nel", MESHES)
+def test_connectivity_bit_exact(nel):
+    L = _lib()
+    dim = len(nel)
+    ref = omesh.elements_3d(*nel) if dim == 3 else omesh.elements_2d(*nel)
+    out = torch.empty(ref.shape, dtype=torch.int32, device="cuda")
+    L.call("et_generate_elements", dim, L.nel_arg(nel), L.ptr(out), L.stream())
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("nel", MESHES)
+@pytest.mark.parametrize("with_cons", [False, True])
+def test_matvec_f64(nel, with_cons):
+    L = _lib()
+    p = _problem(nel, seed=1)
+    cons = p["cons"] if with_cons else None
+    ref = oops.matvec(p["K0"], p["el"], p["E"], p["u"], cons)
+    y = torch.empty(p["n_dof"], dtype=torch.float64, device="cuda")
+    ct = dev(cons.astype(np.uint8)) if with_cons else None
+    L.call("et_matvec_f64", p["dim"], L.nel_arg(nel), L.ptr(dev(p["K0"])), L.ptr(dev(p["E"])),
+           L.ptr(ct), L.ptr(dev(p["u"])), L.ptr(y), L.stream())
+    assert rel(y.cpu().numpy(), ref) < 1e-12
+
+
+@pytest.mark.parametrize("nel", [(12, 8), (6, 4, 4)])
+def test_matvec_f32(nel):
+    L = _lib()
+    p = _problem(nel, seed=2)
+    ref = oops.matvec(p["K0"], p["el"], p["E"], p["u"], p["cons"])
+    y = torch.empty(p["n_dof"], dtype=torch.float32, device="cuda")
+    L.call("et_matvec_f32", p["dim"], L.nel_arg(nel), L.ptr(dev(p["K0"], torch.float32)),
+           L.ptr(dev(p["E"], torch.float32)), L.ptr(dev(p["cons"].astype(np.uint8))),
+           L.ptr(dev(p["u"], torch.float32)), L.ptr(y), L.stream())
+    assert rel(y.cpu().numpy(), ref) < 1e-5
+
+
+@pytest.mark.parametrize("nel", MESHES)
+def test_diagonal_energy_sensitivity(nel):
+    L = _lib()
+    p = _problem(nel, seed=3)
+    ct = dev(p["cons"].astype(np.uint8))
+    d = torch.empty(p["n_dof"], dtype=torch.float64, device="cuda")
+    L.call("et_diagonal_f64", p["dim"], L.nel_arg(nel), L.ptr(dev(p["K0"])), L.ptr(dev(p["E"])),
+           L.ptr(ct), L.ptr(d), L.stream())
+    assert rel(d.cpu().numpy(), oops.diagonal(p["K0"], p["el"], p["E"], p["cons"], p["n_dof"])) < 1e-12
+    dk = torch.empty(p["el"].shape[0], dtype=torch.float64, device="cuda")
+    L.call("et_elem_energy_f64", p["dim"], L.nel_arg(nel), L.ptr(dev(p["K0"])), L.ptr(dev(p["u"])),
+           L.ptr(dk), L.stream())
+    ref_dk = oops.process_dk(p["K0"], p["el"], p["u"])
+    assert rel(dk.cpu().numpy(), ref_dk) < 1e-12
+    xt = np.random.default_rng(4).uniform(0.0, 1.0, p["el"].shape[0])
+    xt[:3] = [0.0, 1e-12, 1.0]
+    s = torch.empty_like(dk)
+    L.call("et_sensitivity_f64", p["dim"], L.nel_arg(nel), L.ptr(dev(p["K0"])), L.ptr(dev(p["u"])),
+           L.ptr(dev(xt)), 3.0, 1e-9, L.ptr(s), L.stream())
+    mat = otop.SingleMaterial(void=1e-9, penalty=3.0)
+    assert rel(s.cpu().numpy(), mat.grad(xt) * ref_dk) < 1e-12
+    E = torch.empty_like(dk)
+    L.call("et_simp_forward_f64", xt.shape[0], L.ptr(dev(xt)), 3.0, 1e-9, 0, L.ptr(E), L.stream())
+    assert rel(E.cpu().numpy(), mat.forward(xt)) < 1e-15
+
+
+@pytest.mark.parametrize("nel,rmin,spacing", [((12, 8), 1.5, (1, 1)), ((20, 7), 2.6, (1, 1)),
+                                              ((9, 7, 5), 2.0, (1, 1, 1)),
+                                              ((8, 6, 7), 2.9, (1, 1, 1)),
+                                              ((10, 6, 4), 2.2, (1.0, 0.5, 2.0))])
+def test_filter(nel, rmin, spacing):
+    L = _lib()
+    dim = len(nel)
+    offs, w = ofilt.offsets_weights(rmin, spacing)
+    norm = ofilt.normalizer(nel, offs, w)
+    n = int(np.prod(nel))
+    x = np.random.default_rng(5).uniform(0, 1, n)
+    o_t, w_t = dev(offs.astype(np.int32)), dev(w)
+    nt = torch.empty(n, dtype=torch.float64, device="cuda")
+    L.call("et_filter_normalizer_f64", dim, L.nel_arg(nel), len(w), L.ptr(o_t), L.ptr(w_t),
+           L.ptr(nt), L.stream())
+    assert rel(nt.cpu().numpy(), norm) < 1e-14
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    L.call("et_filter_fwd_f64", dim, L.nel_arg(nel), len(w), L.ptr(o_t), L.ptr(w_t), L.ptr(dev(x)),
+           L.ptr(y), L.stream())
+    assert rel(y.cpu().numpy(), ofilt.forward(x, nel, offs, w)) < 1e-12
+    L.call("et_filter_adj_f64", dim, L.nel_arg(nel), len(w), L.ptr(o_t), L.ptr(w_t), L.ptr(nt),
+           L.ptr(dev(x)), L.ptr(y), L.stream())
+    assert rel(y.cpu().numpy(), ofilt.transpose(x, nel, offs, w, norm)) < 1e-12
+
+
+@pytest.mark.parametrize("nel", [(8, 4), (12, 6), (8, 4, 4), (6, 4, 8), (7, 5, 3)])
+def test_transfers(nel):
+    L = _lib()
+    dim = len(nel)
+    dof = dim
+    nf = int(np.prod(np.array(nel) + 1)) * dof
+    ncn = int(np.prod(np.array(nel) // 2 + 1)) * dof
+    rng = np.random.default_rng(6)
+    vf, vc = rng.standard_normal(nf), rng.standard_normal(ncn)
+    out_c = torch.empty(ncn, dtype=torch.float64, device="cuda")
+    L.call("et_restrict_f64", dim, L.nel_arg(nel), dof, L.ptr(dev(vf)), L.ptr(out_c), L.stream())
+    assert rel(out_c.cpu().numpy(), omgm.restrict(vf, nel, dof)) < 1e-14
+    out_f = torch.empty(nf, dtype=torch.float64, device="cuda")
+    L.call("et_prolong_f64", dim, L.nel_arg(nel), dof, L.ptr(dev(vc)), L.ptr(out_f), L.stream())
+    assert rel(out_f.cpu().numpy(), omgm.prolong(vc, nel, dof)) < 1e-14
+
+
+def _stencil_to_dense(S, nel, dof):
+    """Expand the library's block-stencil layout into a dense matrix (test helper)."""
+    dim = len(nel)
+    nn = np.array(nel) + 1
+    N = int(np.prod(nn))
+    NO = 27 if dim == 3 else 9
+    S = S.reshape(NO, dof, dof, N)
+    A = np.zeros((N * dof, N * dof))
+    for n in range(N):
+        if dim == 3:
+            nz, nx = nn[2], nn[0]
+            k, t = n % nz, n // nz
+            i, j = t % nx, t // nx
+        else:
+            nx = nn[0]
+            i, j, k = n % nx, n // nx, 0
+        for o in range(NO):
+            if dim == 3:
+                oi, oj, ok = o // 9 - 1, (o // 3) % 3 - 1, o % 3 - 1
+            else:
+                oi, oj, ok = o // 3 - 1, o % 3 - 1, 0
+            mi, mj, mk = i + oi, j + oj, k + ok
+            if not (0 <= mi < nn[0] and 0 <= mj < nn[1]) or (dim == 3 and not 0 <= mk < nn[2]):
+                assert np.all(S[o, :, :, n] == 0)
+                continue
+            m = (mj * nn[0] + mi) * nn[2] + mk if dim == 3 else mj * nn[0] + mi
+            A[n * dof:(n + 1) * dof, m * dof:(m + 1) * dof] = S[o, :, :, n]
+    return A
+
+
+@pytest.mark.parametrize("nel", [(8, 4), (6, 4, 4), (4, 4, 2)])
+def test_stencil_and_galerkin(nel):
+    L = _lib()
+    p = _problem(nel, seed=7)
+    dim, dof = p["dim"], p["dim"]
+    NO = 27 if dim == 3 else 9
+    N = int(np.prod(np.array(nel) + 1))
+    S = torch.empty(NO * dof * dof * N, dtype=torch.float64, device="cuda")
+    L.call("et_stencil_from_operator_f64", dim, L.nel_arg(nel), L.ptr(dev(p["K0"])), L.ptr(dev(p["E"])),
+           L.ptr(dev(p["cons"].astype(np.uint8))), L.ptr(S), L.stream())
+    A = oops.assemble(p["K0"], p["el"], p["E"], p["cons"], p["n_dof"]).toarray()
+    A_gpu = _stencil_to_dense(S.cpu().numpy(), nel, dof)
+    assert rel(A_gpu, A) < 1e-13
+    # stencil application equals the matrix-free operator
+    y = torch.empty(p["n_dof"], dtype=torch.float64, device="cuda")
+    L.call("et_stencil_apply_f64", dim, L.nel_arg(nel), dof, L.ptr(S), L.ptr(dev(p["u"])), L.ptr(y),
+           L.stream())
+    assert rel(y.cpu().numpy(), A @ p["u"]) < 1e-12
+    # Galerkin
+    nc = tuple(v // 2 for v in nel)
+    Nc = int(np.prod(np.array(nc) + 1))
+    Sc = torch.empty(NO * dof * dof * Nc, dtype=torch.float64, device="cuda")
+    L.call("et_galerkin_f64", dim, L.nel_arg(nel), dof, L.ptr(S), L.ptr(Sc), L.stream())
+    R = omgm.restriction_matrix(nel, dof).toarray()
+    assert rel(_stencil_to_dense(Sc.cpu().numpy(), nc, dof), R @ A @ R.T) < 1e-12
+
+
+def test_dot_and_sqdist():
+    L = _lib()
+    rng = np.random.default_rng(8)
+    for n in (1, 255, 4097, 1_000_003):
+        a, b = rng.standard_normal(n), rng.standard_normal(n)
+        out = torch.zeros(1, dtype=torch.float64, device="cuda")
+        L.call("et_dot_f64", n, L.ptr(dev(a)), L.ptr(dev(b)), L.ptr(out), L.stream())
+        assert abs(out.item() - a @ b) <= 1e-12 * np.abs(a * b).sum()
+        L.call("et_sqdist_f64", n, L.ptr(dev(a)), L.ptr(dev(b)), L.ptr(out), L.stream())
+        assert abs(out.item() - np.sum((a - b) ** 2)) <= 1e-12 * np.sum((a - b) ** 2)
+
+
+@pytest.mark.parametrize("n,seed", [(4800, 0), (100_000, 1), (7, 2)])
+def test_oc_update(n, seed):
+    L = _lib()
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.01, 1.0, n)
+    dc = -rng.uniform(0.0, 2.0, n) * rng.uniform(0, 1, n) ** 3
+    mat = otop.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=0.4)
+    ref, steps = otop.oc_update(x, dc, 1.0, float(n), 0.4, 0.2, mat)
+    work = torch.empty(int(L.lib().et_oc_work_size(n)) // 8 + 1, dtype=torch.float64, device="cuda")
+    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    st = ctypes.c_int(0)
+    L.call("et_oc_update_f64", n, L.ptr(dev(x)), L.ptr(dev(dc)), 1.0, float(n), 0.4, 0.2, L.ptr(out),
+           L.ptr(work), ctypes.byref(st), L.stream())
+    assert st.value == steps
+    assert rel(out.cpu().numpy(), ref) < 1e-14
+    # all-zero sensitivities: ocP falls back to 1e-3
+    ref0, _ = otop.oc_update(x, np.zeros(n), 1.0, float(n), 0.4, 0.2, mat)
+    L.call("et_oc_update_f64", n, L.ptr(dev(x)), L.ptr(dev(np.zeros(n))), 1.0, float(n), 0.4, 0.2,
+           L.ptr(out), L.ptr(work), ctypes.byref(st), L.stream())
+    assert rel(out.cpu().numpy(), ref0) < 1e-14

@@ -1,0 +1,97 @@
+"""GPU GMG-PCG solver and SIMP loop against the CPU oracle (small meshes)."""
+import numpy as np
+import pytest
+
+from oracle import ops as oops
+from oracle import topopt as otop
+from oracle.solvers import MultiGrid as OMultiGrid
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _edgeto():
+    import edgeto_amd
+    return edgeto_amd
+
+
+def _cantilever_3d(ET, nel, nu=0.3, rmin=2.0):
+    nx, ny, nz = nel
+    mesh = ET.CuStructuredMesh3D(nx, ny, nz, nx, ny, nz,
+                                 stiffness=lambda x0: ET.auto_stiffness(x0, 1.0, nu))
+    kernel = ET.CuStructuredStiffnessKernel(mesh)
+    filt = ET.CuStructuredFilter3D(mesh, rmin)
+    # clamp face x = 0: node (i=0, j, k) -> j*nx1*nz1 + k
+    nx1, ny1, nz1 = nx + 1, ny + 1, nz + 1
+    j, k = np.meshgrid(np.arange(ny1), np.arange(nz1), indexing="ij")
+    clamp = (j * nx1 * nz1 + k).ravel()
+    load = np.array([0 * nx1 * nz1 + nx * nz1 + kk for kk in range(nz1)])  # x = L, y = 0 edge
+    return mesh, kernel, filt, clamp, load
+
+
+def _oracle_problem(nel, nu, rmin, clamp, load, fy):
+    prob = otop.Problem(nel, nu=nu, r_min=rmin)
+    dof = prob.dof
+    for d in range(dof):
+        prob.cons[clamp * dof + d] = True
+    prob.f[load * dof + 1] = fy
+    return prob
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("nel,n_level", [((16, 8, 8), 2), ((24, 12, 12), 2)])
+def test_solve_matches_direct(precision, nel, n_level):
+    ET = _edgeto()
+    mesh, kernel, filt, clamp, load = _cantilever_3d(ET, nel)
+    rng = np.random.default_rng(0)
+    E = rng.uniform(0.05, 1.0, mesh.n_el)
+    kernel.set_rho(torch.tensor(E, device="cuda"))
+    tp = ET.CuTopOpt(mesh, ET.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=0.3), kernel,
+                     None, filt, verbose=False)
+    tp.add_BC_nodal(clamp, np.array([[1, 1, 1]]))
+    tp.add_F_nodal(load, np.array([[0.0, -1.0 / len(load), 0.0]]))
+    solver = ET.CuMultiGrid(mesh, kernel, tol=1e-10, n_smooth=2, n_level=n_level, precision=precision)
+    x, res = solver.solve(tp.f, use_last=False)
+    assert res < 1e-10
+    prob = _oracle_problem(nel, 0.3, 2.0, clamp, load, -1.0 / len(load))
+    A = oops.assemble(prob.K0, prob.elements, E, prob.cons, prob.n_dof)
+    import scipy.sparse.linalg as spla
+    xd = spla.spsolve(A.tocsc(), prob.f)
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(prob.f - A @ xg) / np.linalg.norm(prob.f) < 2e-10
+    assert np.linalg.norm(xg - xd) / np.linalg.norm(xd) < 1e-7
+    if precision == "fp64":
+        # same algorithm as the oracle's GPU-semantics MultiGrid: PCG iteration counts agree
+        om = OMultiGrid(prob.nel, prob.dof, tol=1e-10, n_smooth=2, n_level=n_level, mode="gpu")
+        om.solve(A, prob.f, use_last=False)
+        assert abs(om.last_iters - solver.last_iterations) <= 1
+
+
+def test_simp_trajectory_2d():
+    """Short 2-D MBB run (config-1 setup at reduced size) vs the oracle, GPU omega semantics."""
+    ET = _edgeto()
+    nelx, nely = 40, 20
+    mesh = ET.CuStructuredMesh2D(nelx, nely, nelx, nely,
+                                 stiffness=lambda x0: ET.auto_stiffness(x0, 1.0, 0.3))
+    kernel = ET.CuStructuredStiffnessKernel(mesh)
+    filt = ET.CuStructuredFilter2D(mesh, 1.5)
+    mat = ET.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=0.5)
+    solver = ET.CuMultiGrid(mesh, kernel, tol=1e-8, n_smooth=3, n_level=2)
+    tp = ET.CuTopOpt(mesh, mat, kernel, solver, filt, max_iter=10, ch_tol=0, fun_tol=0, verbose=False)
+    nx1 = nelx + 1
+    left = np.arange(nely + 1) * nx1
+    tp.add_BC_nodal(left, np.array([[1, 0]]))
+    tp.add_BC_nodal(np.array([nelx]), np.array([[0, 1]]))
+    tp.add_F_nodal(np.array([nely * nx1]), np.array([[0.0, -1.0]]))
+    x, flag, hist = tp.optimize(save_comp_history=True, rho_init=np.full(mesh.n_el, 0.5))
+    prob = otop.Problem((nelx, nely), nu=0.3, r_min=1.5)
+    prob.cons[left * 2] = True
+    prob.cons[nelx * 2 + 1] = True
+    prob.f[nely * nx1 * 2 + 1] = -1.0
+    omat = otop.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=0.5)
+    osol = OMultiGrid(prob.nel, 2, tol=1e-8, n_smooth=3, n_level=2, mode="gpu")
+    xo, _, ohist = otop.optimize(prob, omat, osol, np.full(prob.n_el, 0.5), 10, ch_tol=0, fun_tol=0)
+    comp_gpu = np.array([tp.iteration_log[0]["comp"]] + hist["comp_history"])
+    comp_ref = np.array(ohist["comp"])
+    assert np.max(np.abs(comp_gpu - comp_ref) / comp_ref) < 1e-6
+    assert np.max(np.abs(x.cpu().numpy() - xo)) < 1e-4
