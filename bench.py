@@ -1,0 +1,200 @@
+"""Headline benchmark: SIMP OC iterations per second on one MI355X.
+
+Workload (default): 3-D cantilever of 256 x 128 x 128 unit H8 elements
+(4 194 304 elements, 12 830 211 DOF), E0 = 1, Emin = 1e-9, nu = 0.3, volume
+fraction 0.3, penalty 3, hat filter rmin = 2.0, x0 = 0.3, face x = 0 clamped,
+unit total downward load spread over the bottom edge at x = L, 5-grid GMG
+(n_level = 4) V(2,2)-PCG to relative residual 1e-8.  One step = one OC
+iteration: OC update, filter, SIMP map, GMG-PCG solve (hierarchy rebuilt),
+compliance, sensitivity, filter transpose.
+
+Both legs run the same steps from the same start: the mixed leg (FP32 V-cycle
+inside FP64 PCG) gives `value`; the plain FP64 leg is reported beside it with
+the compliance agreement.  `roofline` is the dominant kernel class of the
+mixed leg, timed with HIP events on the solver stream over the timed region.
+`cpu_baseline` times the NumPy/SciPy oracle on a bounded sample (see
+`cpu_baseline.sample`).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "edge-to_amd"))
+
+METRIC = "SIMP iters/s, 3D 12.8M-DOF H8: mixed vs FP64 GMG-PCG at rel. compliance 1e-6"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "c3": dict(nel=(256, 128, 128), n_level=4),
+    "c2": dict(nel=(96, 48, 48), n_level=3),
+}
+
+
+def build_problem(ET, nel, n_level, precision):
+    import torch
+    nx, ny, nz = nel
+    mesh = ET.CuStructuredMesh3D(nx, ny, nz, nx, ny, nz,
+                                 stiffness=lambda x0: ET.auto_stiffness(x0, 1.0, 0.3))
+    kernel = ET.CuStructuredStiffnessKernel(mesh)
+    filt = ET.CuStructuredFilter3D(mesh, 2.0)
+    mat = ET.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=0.3)
+    solver = ET.CuMultiGrid(mesh, kernel, tol=1e-8, n_smooth=2, n_level=n_level, precision=precision)
+    tp = ET.CuTopOpt(mesh, mat, kernel, solver, filt, max_iter=0, ch_tol=0, fun_tol=0, verbose=False)
+    nx1, ny1, nz1 = nx + 1, ny + 1, nz + 1
+    j, k = np.meshgrid(np.arange(ny1), np.arange(nz1), indexing="ij")
+    tp.add_BC_nodal((j * nx1 * nz1 + k).ravel(), np.array([[1, 1, 1]]))
+    load = np.array([nx * nz1 + kk for kk in range(nz1)])  # x = L, y = 0, all z
+    tp.add_F_nodal(load, np.array([[0.0, -1.0 / len(load), 0.0]]))
+    torch.cuda.synchronize()
+    return mesh, tp
+
+
+def run_leg(ET, wl, precision, warmup, steps, profile=False):
+    """Initial solve + warmup OC iterations, then `steps` timed OC iterations."""
+    import torch
+    from edgeto_amd import _lib
+    mesh, tp = build_problem(ET, wl["nel"], wl["n_level"], precision)
+    x = tp.material_model(torch.full((mesh.n_el,), 0.3, dtype=torch.float64, device="cuda"), 0, plain=True)
+    tp.kernel.set_rho(x)
+    comp, df, res = tp.solve_and_process(x, 0)
+    for i in range(warmup):
+        x = tp.update_design(x, df)
+        comp, df, res = tp.solve_and_process(x, i)
+    h = tp.solver._handle
+    if profile:
+        _lib.call("et_mg_profile", h, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pcg, per_step = [], []
+    for i in range(steps):
+        ts = time.perf_counter()
+        x = tp.update_design(x, df)
+        comp, df, res = tp.solve_and_process(x, warmup + i)   # compliance read back: host sync
+        per_step.append(time.perf_counter() - ts)
+        pcg.append(tp.solver.last_iterations)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = None
+    if profile:
+        ms = (ctypes.c_double * 10)()
+        cnt = (ctypes.c_int64 * 10)()
+        _lib.call("et_mg_profile_read", h, ctypes.cast(ms, ctypes.c_void_p), ctypes.cast(cnt, ctypes.c_void_p))
+        _lib.call("et_mg_profile", h, 0)
+        prof = {k: (ms[i], cnt[i]) for i, k in enumerate(_lib.PROF_KINDS)}
+    return dict(elapsed=elapsed, per_step=per_step, comp=comp, res=res, pcg=pcg, prof=prof,
+                n_dof=tp.kernel.shape[0], n_el=mesh.n_el)
+
+
+def roofline(prof, n_dof, n_el):
+    """Dominant kernel class of the mixed leg and its achieved algorithmic bandwidth."""
+    s = 4  # FP32 V-cycle vectors / E
+    mask = n_dof  # one byte per DOF
+    bytes_per = {  # algorithmic bytes per launch
+        "l0_jacobi": s * (4 * n_dof + n_el) + mask,       # read x, b, D^-1, E; write y
+        "l0_residual": s * (3 * n_dof + n_el) + mask,     # read x, b, E; write r
+        "l0_jacobi_add": s * (5 * n_dof + n_el) + mask,   # + read x_prev
+        "matvec64": 8 * (3 * n_dof + n_el) + mask,        # read x, b(res only) ~ read x, E; write y
+    }
+    cand = {k: v for k, v in prof.items() if k in bytes_per and v[1] > 0}
+    name = max(cand, key=lambda k: cand[k][0])
+    ms, count = cand[name]
+    avg_s = ms / count * 1e-3
+    achieved = bytes_per[name] / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": name, "avg_us": round(avg_s * 1e6, 2), "launches": count,
+            "bytes_per_launch": bytes_per[name]}
+
+
+def cpu_baseline(nel_metric, sample_nel=(48, 24, 24), n_level=2):
+    """Oracle (NumPy/SciPy, FP64, one thread) on a bounded sample, scaled by element count."""
+    from threadpoolctl import threadpool_limits
+    sys.path.insert(0, ROOT)
+    from oracle import topopt
+    from oracle.solvers import MultiGrid
+    with threadpool_limits(limits=1):
+        nx, ny, nz = sample_nel
+        p = topopt.Problem(sample_nel, nu=0.3, r_min=2.0)
+        nx1, ny1, nz1 = nx + 1, ny + 1, nz + 1
+        j, k = np.meshgrid(np.arange(ny1), np.arange(nz1), indexing="ij")
+        clamp = (j * nx1 * nz1 + k).ravel()
+        for d in range(3):
+            p.cons[clamp * 3 + d] = True
+        load = np.array([nx * nz1 + kk for kk in range(nz1)])
+        p.f[load * 3 + 1] = -1.0 / len(load)
+        mat = topopt.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=0.3)
+        solver = MultiGrid(p.nel, 3, tol=1e-8, n_smooth=2, n_level=n_level, mode="gpu")
+        x = np.full(p.n_el, 0.3)
+        c, dr, _, _ = topopt.solve_and_process(p, x, mat, solver)   # initial solve (untimed)
+        t = time.perf_counter()
+        x, _ = topopt.oc_update(x, dr, p.A, p.volume, 0.3, 0.2, mat)
+        topopt.solve_and_process(p, x, mat, solver)
+        dt = time.perf_counter() - t
+    scale = np.prod(nel_metric) / np.prod(sample_nel)
+    return {"value": round(1.0 / (dt * scale), 6), "unit": "SIMP iters/s", "cores": 1, "kind": "port",
+            "sample": f"1 OC iteration of the oracle on a {sample_nel[0]}x{sample_nel[1]}x{sample_nel[2]} "
+                      f"H8 cantilever (same BCs/load/filter, {n_level + 1}-grid GMG, tol 1e-8): {dt:.2f} s, "
+                      f"scaled by the element-count ratio {scale:.0f} to {nel_metric[0]}x{nel_metric[1]}x{nel_metric[2]}",
+            "sample_seconds": round(dt, 3),
+            "host_cpus": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--legs", default="mixed,fp64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); the multi-GPU path is "
+              "a design (see DESIGN.md) and is not built.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import edgeto_amd as ET
+    wl = WORKLOADS[args.workload]
+    legs = {}
+    for leg in args.legs.split(","):
+        print(f"[bench] leg {leg}: warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
+        legs[leg] = run_leg(ET, wl, leg, args.warmup, args.steps, profile=(leg == "mixed"))
+        print(f"[bench] leg {leg}: {args.steps / legs[leg]['elapsed']:.3f} it/s, "
+              f"pcg {legs[leg]['pcg']}, comp {legs[leg]['comp']:.12e}", file=sys.stderr, flush=True)
+    main_leg = legs.get("mixed") or legs[list(legs)[0]]
+    value = args.steps / main_leg["elapsed"]
+    out = {"metric": METRIC, "value": round(value, 4), "unit": "SIMP iters/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(main_leg["elapsed"] / args.steps * 1e3, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 V-cycle + f64 PCG" if "mixed" in legs else "f64",
+           "data": "synthetic (closed-form cantilever BCs and load, uniform x0 = 0.3)",
+           "config": {"workload": f"3-D H8 cantilever {'x'.join(map(str, wl['nel']))}, "
+                                  f"{wl['n_level'] + 1}-grid GMG V(2,2)-PCG tol 1e-8, rmin 2.0, vf 0.3",
+                      "n_dof": main_leg["n_dof"], "n_elem": main_leg["n_el"],
+                      "pcg_iterations": main_leg["pcg"], "parallelism": "single GPU"}}
+    if "fp64" in legs and "mixed" in legs:
+        f = legs["fp64"]
+        out["fp64"] = {"value": round(args.steps / f["elapsed"], 4), "pcg_iterations": f["pcg"],
+                       "compliance": f["comp"]}
+        out["speedup_vs_fp64"] = round(f["elapsed"] / main_leg["elapsed"], 3)
+        out["compliance_rel_diff"] = abs(main_leg["comp"] - f["comp"]) / abs(f["comp"])
+    if main_leg["prof"] is not None:
+        out["roofline"] = roofline(main_leg["prof"], main_leg["n_dof"], main_leg["n_el"])
+        out["kernel_classes_ms"] = {k: round(v[0], 2) for k, v in main_leg["prof"].items()}
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle sample)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(wl["nel"])
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

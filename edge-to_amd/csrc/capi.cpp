@@ -6,6 +6,7 @@
 
 #include "blas.hpp"
 #include "common.hpp"
+#include "galerkin_elem.hpp"
 #include "solver.hpp"
 #include "stiffness.hpp"
 
@@ -17,9 +18,9 @@ const char* last_error() { return g_err.c_str(); }
 
 using et::Grid;
 
-#define ET_GUARD(body)                                   \
+#define ET_GUARD(...)                                    \
   try {                                                  \
-    body                                                 \
+    __VA_ARGS__                                          \
   } catch (const std::bad_alloc&) {                      \
     et::set_error("host allocation failed");             \
     return ET_ENOMEM;                                    \
@@ -32,6 +33,17 @@ extern "C" {
 
 const char* et_last_error(void) { return et::last_error(); }
 const char* et_version(void) { return "edgeto_amd 0.1 (gfx950)"; }
+
+int et_tune_set(int param, int value) {
+  ET_GUARD({
+    ET_REQUIRE(param == ET_TUNE_TILED_JC, "et_tune_set: unknown parameter");
+    ET_REQUIRE(value >= 1 && value <= 1024, "et_tune_set: value out of range");
+    et::set_tiled_jc(value);
+    return ET_OK;
+  })
+}
+
+int et_tune_get(int param) { return param == ET_TUNE_TILED_JC ? et::get_tiled_jc() : -1; }
 
 int et_generate_elements(int dim, const int* nel, int32_t* elements, hipStream_t stream) {
   ET_GUARD({
@@ -175,6 +187,37 @@ int et_galerkin_f64(int dim, const int* nel, int dof, const double* S_fine, doub
   })
 }
 
+int et_galerkin_elements_f64(int dim, const int* nel, const double* K0_host, const double* E,
+                             const uint8_t* cons, double* S_coarse, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(K0_host && E && cons && S_coarse, "et_galerkin_elements_f64: null pointer");
+    ET_REQUIRE(et::galerkin_elem_supported(g), "et_galerkin_elements_f64: element counts must be even");
+    const int nd = (dim == 3 ? 8 : 4) * dim;
+    const std::vector<double> M = et::galerkin_child_tables(dim, K0_host);
+    double *Mt = nullptr, *K0 = nullptr;
+    uint8_t* flag = nullptr;
+    int st = ET_OK;
+    if (hipMalloc(&Mt, M.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&K0, nd * nd * sizeof(double)) != hipSuccess || hipMalloc(&flag, g.n_elem()) != hipSuccess) {
+      et::set_error("et_galerkin_elements_f64: device allocation failed");
+      st = ET_ENOMEM;
+    }
+    if (st == ET_OK && (hipMemcpy(Mt, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(K0, K0_host, nd * nd * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)) {
+      et::set_error("et_galerkin_elements_f64: copy failed");
+      st = ET_EHIP;
+    }
+    if (st == ET_OK) st = et::galerkin_from_elements(g, K0, Mt, E, cons, flag, S_coarse, stream);
+    if (hipStreamSynchronize(stream) != hipSuccess && st == ET_OK) st = ET_EHIP;
+    (void)hipFree(Mt);
+    (void)hipFree(K0);
+    (void)hipFree(flag);
+    return st;
+  })
+}
+
 int et_stencil_apply_f64(int dim, const int* nel, int dof, const double* S, const double* x,
                          double* y, hipStream_t stream) {
   ET_GUARD({
@@ -252,6 +295,23 @@ int et_mg_last_timings(et_mg* handle, double* setup_ms, double* pcg_ms) {
     if (setup_ms) *setup_ms = handle->impl.t_setup_ms;
     if (pcg_ms) *pcg_ms = handle->impl.t_pcg_ms;
     return ET_OK;
+  })
+}
+
+int et_mg_profile(et_mg* handle, int enable) {
+  ET_GUARD({
+    ET_REQUIRE(handle, "et_mg_profile: null handle");
+    ET_CHECK_HIP(hipStreamSynchronize(handle->impl.stream));
+    handle->impl.prof.reset();
+    handle->impl.prof.on = enable != 0;
+    return ET_OK;
+  })
+}
+
+int et_mg_profile_read(et_mg* handle, double* ms, int64_t* launches) {
+  ET_GUARD({
+    ET_REQUIRE(handle && ms && launches, "et_mg_profile_read: null pointer");
+    return handle->impl.profile_read(ms, launches);
   })
 }
 

@@ -1,0 +1,207 @@
+// LDS-tiled, node-centric H8 operator for the fine grid (device code).
+//
+// A workgroup owns a strip of 256 consecutive nodes of one x-z node plane
+// (plane index p = i*nz + k, contiguous in memory) and marches over JC
+// consecutive y planes.  Per plane it stages in LDS: the u values of the strip
+// plus a halo of nz+1 nodes on each side (SoA per component, constrained
+// entries zeroed), in a ring of three planes, and the E values of the element
+// columns the strip touches, in a ring of two element planes.  Each thread
+// then forms
+//     y_n = sum_a E_a * sum_o K0[a, a+o] u_{n+o}
+// visiting the 27 neighbours o once (81 LDS reads) and keeping one partial
+// 3-vector per adjacent element a.  Every output is written once: no
+// atomics, fixed summation order.
+//
+// Element matrix: a structured-grid element is an axis-aligned box, so K0 is
+// invariant under the reflections of the box.  Reflecting by the corner bits
+// of a maps corner a to 0 and flips the sign of the reflected displacement
+// components:  K0[a, b] = R_a K0[0, a^b] R_a  (corner bits XORed, R_a the
+// diagonal sign matrix).  Only the first block row of K0 (8 blocks of 3x3,
+// 72 values) is therefore needed; it is read with scalar loads and the signs
+// are compile-time constants folded into the FMAs.  For a fixed neighbour offset
+// o every pair (a, a+o) uses the same base block, the one whose XOR pattern
+// marks the non-zero components of o.
+#pragma once
+#include "common.hpp"
+
+namespace et {
+
+constexpr int kStrip = 256;
+
+// Base-block entry (r, block m, c) = K0[r][3m + c]: wave-uniform, compile-time
+// offsets, so it is fetched with scalar loads into SGPRs.
+#define ET_TILED_K(r, m, c) K0[(r) * 24 + 3 * (m) + (c)]
+
+enum TiledMode { TM_Y = 0, TM_RES = 1, TM_JAC = 2, TM_JAC_ADD = 3 };
+
+struct TiledArgs {
+  Grid g;
+  int64_t e_sx, e_sy;  // element index = ex*e_sx + ey*e_sy + ez
+  int jc;              // y planes per workgroup
+  int halo;            // nz + 1
+  int u_span;          // kStrip + 2*halo
+  int e_span;          // element slots per element plane
+};
+
+__host__ inline TiledArgs tiled_args(const Grid& g, int64_t e_sx, int64_t e_sy, int jc) {
+  TiledArgs a;
+  a.g = g;
+  a.e_sx = e_sx;
+  a.e_sy = e_sy;
+  a.jc = jc;
+  a.halo = g.nz() + 1;
+  a.u_span = kStrip + 2 * a.halo;
+  // x columns spanned by a strip (+1 on the low side for the elements behind it)
+  const int cols = kStrip / g.nz() + 3;
+  a.e_span = cols * g.ez;
+  return a;
+}
+
+template <typename T>
+__host__ inline size_t tiled_lds_bytes(const TiledArgs& a) {
+  return (size_t)3 * 3 * a.u_span * sizeof(T) + (size_t)2 * a.e_span * sizeof(T);
+}
+
+// offset bit of H8 corner a along axis d (0 = x, 1 = y, 2 = z)
+__host__ __device__ constexpr int h8_bit(int a, int d) { return d == 0 ? hx(a) : (d == 1 ? hy(a) : hz(a)); }
+
+// corner index of the H8 node with offset bits (bx, by, bz)
+__host__ __device__ constexpr int h8_corner(int bx, int by, int bz) {
+  return (bz ? 4 : 0) + (by ? (bx ? 2 : 3) : (bx ? 1 : 0));
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void op0_tiled_kernel(const TiledArgs A, const T* __restrict__ K0,
+                                                        const T* __restrict__ E,
+                                                        const uint8_t* __restrict__ cons,
+                                                        const T* __restrict__ x,
+                                                        const T* __restrict__ b,
+                                                        const T* __restrict__ Dinv,
+                                                        const double* __restrict__ omega,
+                                                        double wscale, const T* __restrict__ xadd,
+                                                        T* __restrict__ y) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* su = reinterpret_cast<T*>(lds_raw);        // [3 planes][3 comps][u_span]
+  T* se = su + 9 * A.u_span;                    // [2 planes][e_span]
+  const Grid& g = A.g;
+  const int nx = g.nx(), ny = g.ny(), nz = g.nz();
+  const int64_t plane = (int64_t)nx * nz;
+  const int64_t p0 = (int64_t)blockIdx.x * kStrip;
+  const int j0 = blockIdx.y * A.jc;
+  const int j1 = min(j0 + A.jc, ny);
+  const int tid = threadIdx.x;
+  const int64_t p = p0 + tid;
+  const bool active = p < plane;
+  const int i = (int)(p / nz), k = (int)(p % nz);
+  const int ex0 = (int)(p0 / nz) - 1;  // first element column staged
+  const int64_t ubase = p0 - A.halo;   // plane index of LDS slot 0
+
+  auto load_u = [&](int jp) {
+    T* dst = su + (size_t)((jp + 3) % 3) * 3 * A.u_span;
+    const bool in = jp >= 0 && jp < ny;
+    const int64_t nvals = (int64_t)A.u_span * 3;
+    for (int64_t t = tid; t < nvals; t += blockDim.x) {
+      const int64_t q = ubase + t / 3;
+      const int c = (int)(t % 3);
+      T v = T(0);
+      if (in && q >= 0 && q < plane) {
+        const int64_t d = ((int64_t)jp * plane + q) * 3 + c;
+        v = (cons != nullptr && cons[d]) ? T(0) : x[d];
+      }
+      dst[c * A.u_span + t / 3] = v;
+    }
+  };
+  auto load_e = [&](int ey) {
+    T* dst = se + (size_t)((ey + 2) % 2) * A.e_span;
+    const bool in = ey >= 0 && ey < g.ey;
+    for (int t = tid; t < A.e_span; t += blockDim.x) {
+      const int ex = ex0 + t / g.ez, ez = t % g.ez;
+      T v = T(0);
+      if (in && ex >= 0 && ex < g.ex) v = E[(int64_t)ex * A.e_sx + (int64_t)ey * A.e_sy + ez];
+      dst[t] = v;
+    }
+  };
+
+  load_u(j0 - 1);
+  load_u(j0);
+  load_e(j0 - 1);
+  for (int j = j0; j < j1; ++j) {
+    load_u(j + 1);
+    load_e(j);
+    __syncthreads();
+    if (active) {
+      T s[8][3];
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) s[a][r] = T(0);
+      const int lp = tid + A.halo;  // LDS slot of this node
+      // Offsets are visited grouped by their base block (XOR pattern mb): one
+      // 3x3 block is live at a time, each neighbour is read once.
+#pragma unroll
+      for (int mb = 0; mb < 8; ++mb) {
+        const int ax = mb & 1, ay = (mb >> 1) & 1, az = (mb >> 2) & 1;
+        T kv[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) kv[r][c] = ET_TILED_K(r, h8_corner(ax, ay, az), c);
+#pragma unroll
+        for (int sc = 0; sc < 8; ++sc) {
+          const int sx = sc & 1, sy = (sc >> 1) & 1, sz = (sc >> 2) & 1;
+          if ((!ax && sx) || (!ay && sy) || (!az && sz)) continue;
+          const int di = ax ? (sx ? 1 : -1) : 0;
+          const int dj = ay ? (sy ? 1 : -1) : 0;
+          const int dk = az ? (sz ? 1 : -1) : 0;
+          const T* pl = su + (size_t)((j + dj + 3) % 3) * 3 * A.u_span;
+          const int q = lp + di * nz + dk;
+          // At k = 0 / k = nz-1 the slot q +- 1 wraps into the neighbouring x column;
+          // such values only feed elements outside the grid, whose E is zero.
+          const T u[3] = {pl[q], pl[A.u_span + q], pl[2 * A.u_span + q]};
+#pragma unroll
+          for (int a = 0; a < 8; ++a) {
+            const int bx = hx(a) + di, by = hy(a) + dj, bz = hz(a) + dk;
+            if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+            // (R_a K R_a)[r][c] = sign(a, r) sign(a, c) K[r][c]: a compile-time sign
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                const bool neg = (h8_bit(a, r) ^ h8_bit(a, c)) != 0;
+                s[a][r] = neg ? fma(-kv[r][c], u[c], s[a][r]) : fma(kv[r][c], u[c], s[a][r]);
+              }
+          }
+        }
+      }
+      T acc[3] = {T(0), T(0), T(0)};
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        const int ex = i - hx(a), ey = j - hy(a), ez = k - hz(a);
+        const bool zin = ez >= 0 && ez < g.ez;
+        const T Ea = se[(size_t)((ey + 2) % 2) * A.e_span + (ex - ex0) * g.ez + (zin ? ez : 0)];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) acc[r] += (zin ? Ea : T(0)) * s[a][r];
+      }
+      const int64_t n = (int64_t)j * plane + p;
+      const T w = MODE >= TM_JAC ? (T)(*omega * wscale) : T(0);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const int64_t d = n * 3 + r;
+        const T xv = x[d];
+        const T ax = (cons != nullptr && cons[d]) ? xv : acc[r];
+        T v;
+        if (MODE == TM_Y)
+          v = ax;
+        else if (MODE == TM_RES)
+          v = b[d] - ax;
+        else
+          v = xv + w * Dinv[d] * (b[d] - ax);
+        if (MODE == TM_JAC_ADD) v += xadd[d];
+        y[d] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace et

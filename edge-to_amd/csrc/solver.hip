@@ -16,6 +16,7 @@
 #include "blas.hpp"
 #include "coarse.hpp"
 #include "common.hpp"
+#include "galerkin_elem.hpp"
 #include "solver.hpp"
 #include "stiffness.hpp"
 
@@ -223,10 +224,22 @@ __global__ void set_scalar_kernel(double* sc, int slot, double v) { sc[slot] = v
 
 // ------------------------------------------------------------ host side ----
 
+static int op_kind(int level, int mode) {
+  if (level > 0) return ET_PROF_COARSE_OPS;
+  return mode == OP_RES ? ET_PROF_L0_RES : (mode == OP_JAC_ADD ? ET_PROF_L0_JACADD : ET_PROF_L0_JAC);
+}
+
 template <typename T>
 static int op_launch(const MGLevel<T>& L, const MGShared& sh, int mode, const T* x, const T* b,
                      const double* omega, double wscale, const T* xadd, T* y, hipStream_t s) {
   const Grid& g = L.g;
+  sh.prof->begin(op_kind(L.level, mode), s);
+  if (L.level == 0 && sh.tiled) {
+    const int st = op0_tiled<T>(mode, g, (int64_t)g.ey * g.ez, g.ez, sh.K0<T>(), sh.E<T>(), sh.cons,
+                                x, b, L.Dinv, omega, wscale, xadd, y, s);
+    sh.prof->end(s);
+    return st;
+  }
   const unsigned grid = grid_for(g.n_nodes());
 #define ET_OP0(DIM, MODE)                                                                   \
   op0_kernel<T, DIM, MODE><<<grid, kBlock, 0, s>>>(g, sh.K0<T>(), sh.E<T>(), sh.cons, x, b, \
@@ -256,6 +269,7 @@ static int op_launch(const MGLevel<T>& L, const MGShared& sh, int mode, const T*
 #undef ET_DISPATCH
 #undef ET_OPS
 #undef ET_OP0
+  sh.prof->end(s);
   ET_CHECK_LAUNCH();
   return ET_OK;
 }
@@ -263,14 +277,21 @@ static int op_launch(const MGLevel<T>& L, const MGShared& sh, int mode, const T*
 template <typename T>
 int MGHierarchy<T>::vcycle(int l, MGShared& sh, hipStream_t s) {
   MGLevel<T>& L = lv[l];
-  if (l == n_level) return coarse->solve<T, T>(L.b, L.x, s);
+  if (l == n_level) {
+    sh.prof->begin(ET_PROF_COARSE_SOLVE, s);
+    const int st = coarse->solve<T, T>(L.b, L.x, s);
+    sh.prof->end(s);
+    return st;
+  }
   const double ws = std::pow(1.06, l);
   const double* om = sh.scal + SC_OMEGA;
   const int64_t n = L.n;
   if (n_smooth == 0) {
     ET_CHECK_HIP(hipMemsetAsync(L.x, 0, n * sizeof(T), s));
   } else {
+    sh.prof->begin(l == 0 ? ET_PROF_L0_SCALE : ET_PROF_COARSE_OPS, s);
     scale_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, L.Dinv, L.b, om, ws, L.x);
+    sh.prof->end(s);
     ET_CHECK_LAUNCH();
     for (int it = 1; it < n_smooth; ++it) {
       ET_TRY(op_launch<T>(L, sh, OP_JAC, L.x, L.b, om, ws, nullptr, L.t, s));
@@ -280,9 +301,13 @@ int MGHierarchy<T>::vcycle(int l, MGShared& sh, hipStream_t s) {
   const int passes = (cycle_w && ((w_level_mask >> l) & 1)) ? 2 : 1;
   for (int pass = 0; pass < passes; ++pass) {
     ET_TRY(op_launch<T>(L, sh, OP_RES, L.x, L.b, om, ws, nullptr, L.r, s));
+    sh.prof->begin(ET_PROF_TRANSFER, s);
     ET_TRY(restrict_op<T>(L.g, dof, L.r, lv[l + 1].b, s));
+    sh.prof->end(s);
     ET_TRY(vcycle(l + 1, sh, s));
+    sh.prof->begin(ET_PROF_TRANSFER, s);
     ET_TRY(prolong_op<T>(L.g, dof, lv[l + 1].x, L.e, s));
+    sh.prof->end(s);
     if (n_smooth == 0) {
       axpy_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, L.e, L.x);
       ET_CHECK_LAUNCH();
@@ -336,8 +361,26 @@ template struct MGHierarchy<float>;
 
 // ------------------------------------------------------------ et_mg ----
 
+int MGSolver::profile_read(double* ms, int64_t* launches) {
+  ET_CHECK_HIP(hipStreamSynchronize(stream));
+  for (int k = 0; k < ET_PROF_KINDS; ++k) {
+    ms[k] = 0.0;
+    launches[k] = 0;
+  }
+  for (const auto& r : prof.recs) {
+    float t = 0.f;
+    ET_CHECK_HIP(hipEventElapsedTime(&t, r.a, r.b));
+    if (r.kind >= 0 && r.kind < ET_PROF_KINDS) {
+      ms[r.kind] += t;
+      launches[r.kind] += 1;
+    }
+  }
+  return ET_OK;
+}
+
 int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s) {
   prm = p;
+  sh.prof = &prof;
   stream = s;
   ET_TRY(make_grid(p.dim, p.nel, &g0));
   dof = p.dim;
@@ -354,6 +397,7 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
   }
   n_dof = g0.n_nodes() * dof;
   n_elem = g0.n_elem();
+  sh.tiled = g0.dim == 3 && tiled_supported(g0, sizeof(double));
   const int nd = (p.dim == 3 ? 8 : 4) * dof;
   ET_CHECK_HIP(hipMalloc(&sh.K0d, nd * nd * sizeof(double)));
   ET_CHECK_HIP(hipMalloc(&sh.K0f, nd * nd * sizeof(float)));
@@ -372,10 +416,18 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
   // FP64 stencils: level 0 (fine operator) and every coarse level
   const int NO = p.dim == 3 ? 27 : 9;
   S64.assign(p.n_level + 1, nullptr);
+  elem_galerkin = p.n_level > 0 && galerkin_elem_supported(g0);
+  if (elem_galerkin) {
+    const std::vector<double> M = galerkin_child_tables(p.dim, K0_host);
+    ET_CHECK_HIP(hipMalloc(&Mt, M.size() * sizeof(double)));
+    ET_CHECK_HIP(hipMemcpy(Mt, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
+    ET_CHECK_HIP(hipMalloc(&eflag, (size_t)n_elem));
+  }
   {
     Grid g = g0;
     for (int l = 0; l <= p.n_level; ++l) {
-      if (p.n_level > 0) ET_CHECK_HIP(hipMalloc(&S64[l], (size_t)NO * dof * dof * g.n_nodes() * sizeof(double)));
+      const bool need = p.n_level > 0 && !(l == 0 && elem_galerkin);
+      if (need) ET_CHECK_HIP(hipMalloc(&S64[l], (size_t)NO * dof * dof * g.n_nodes() * sizeof(double)));
       if (l == p.n_level) coarse_grid = g;
       g = g.coarse();
     }
@@ -400,13 +452,14 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
 }
 
 void MGSolver::destroy() {
+  prof.release();
   h64.release();
   h32.release();
   coarse.release();
   for (double* v : S64)
     if (v) (void)hipFree(v);
   for (void* v : {(void*)sh.K0d, (void*)sh.K0f, (void*)sh.scal, (void*)partial, (void*)r, (void*)z,
-                  (void*)p_, (void*)q, (void*)diag64, (void*)sh.Ef})
+                  (void*)p_, (void*)q, (void*)diag64, (void*)sh.Ef, (void*)Mt, (void*)eflag})
     if (v) (void)hipFree(v);
   if (host_scal) (void)hipHostFree(host_scal);
   if (ev0) (void)hipEventDestroy(ev0);
@@ -425,10 +478,13 @@ int MGSolver::setup_levels(MGHierarchy<T>& h) {
   if (L == 0) {
     return ET_EINVAL;  // a direct solve of the fine grid is not supported
   }
-  ET_TRY(stencil_from_operator(g0, sh.K0d, sh.Ed, sh.cons, S64[0], s));
+  if (!elem_galerkin) ET_TRY(stencil_from_operator(g0, sh.K0d, sh.Ed, sh.cons, S64[0], s));
   Grid g = g0;
   for (int l = 1; l <= L; ++l) {
-    ET_TRY(galerkin(g, dof, S64[l - 1], S64[l], s));
+    if (l == 1 && elem_galerkin)
+      ET_TRY(galerkin_from_elements(g0, sh.K0d, Mt, sh.Ed, sh.cons, eflag, S64[1], s));
+    else
+      ET_TRY(galerkin(g, dof, S64[l - 1], S64[l], s));
     g = g.coarse();
     MGLevel<T>& lev = h.lv[l];
     const int64_t N = g.n_nodes();
@@ -453,10 +509,14 @@ int MGSolver::precondition(MGHierarchy<T>& h) {
   // z = V(r): level-0 rhs is the PCG residual, result goes to z
   hipStream_t s = stream;
   MGLevel<T>& L0 = h.lv[0];
+  prof.begin(ET_PROF_PCG_VEC, s);
   convert_kernel<double, T><<<grid_for(n_dof), kBlock, 0, s>>>(n_dof, r, L0.b);
+  prof.end(s);
   ET_CHECK_LAUNCH();
   ET_TRY(h.vcycle(0, sh, s));
+  prof.begin(ET_PROF_PCG_VEC, s);
   convert_kernel<T, double><<<grid_for(n_dof), kBlock, 0, s>>>(n_dof, L0.x, z);
+  prof.end(s);
   ET_CHECK_LAUNCH();
   return ET_OK;
 }
@@ -464,6 +524,13 @@ int MGSolver::precondition(MGHierarchy<T>& h) {
 int MGSolver::apply_A64(const double* x, double* y, int mode, const double* b) {
   hipStream_t s = stream;
   const unsigned grid = grid_for(g0.n_nodes());
+  prof.begin(ET_PROF_MATVEC64, s);
+  if (sh.tiled) {
+    const int st = op0_tiled<double>(mode, g0, (int64_t)g0.ey * g0.ez, g0.ez, sh.K0d, sh.Ed, sh.cons,
+                                     x, b, nullptr, sh.scal, 1.0, nullptr, y, s);
+    prof.end(s);
+    return st;
+  }
   if (g0.dim == 3) {
     if (mode == OP_Y)
       op0_kernel<double, 3, OP_Y><<<grid, kBlock, 0, s>>>(g0, sh.K0d, sh.Ed, sh.cons, x, b, nullptr, sh.scal, 1.0, nullptr, y);
@@ -475,6 +542,7 @@ int MGSolver::apply_A64(const double* x, double* y, int mode, const double* b) {
     else
       op0_kernel<double, 2, OP_RES><<<grid, kBlock, 0, s>>>(g0, sh.K0d, sh.Ed, sh.cons, x, b, nullptr, sh.scal, 1.0, nullptr, y);
   }
+  prof.end(s);
   ET_CHECK_LAUNCH();
   return ET_OK;
 }
@@ -494,6 +562,7 @@ int MGSolver::solve(const double* E, const uint8_t* cons, const double* b, doubl
   sh.Ed = E;
   sh.cons = cons;
   ET_CHECK_HIP(hipEventRecord(ev0, s));
+  prof.begin(ET_PROF_SETUP, s);
   if (prm.precision == 1) {
     convert_kernel<double, float><<<grid_for(n_elem), kBlock, 0, s>>>(n_elem, E, sh.Ef);
     ET_CHECK_LAUNCH();
@@ -501,6 +570,7 @@ int MGSolver::solve(const double* E, const uint8_t* cons, const double* b, doubl
   } else {
     ET_TRY(setup_levels(h64));
   }
+  prof.end(s);
   ET_TRY(coarse.check_info(s));
   ET_CHECK_HIP(hipEventRecord(ev1, s));
   // omega <- min_omega at the start of every solve (GPU semantics)
@@ -520,10 +590,12 @@ int MGSolver::solve(const double* E, const uint8_t* cons, const double* b, doubl
   int status = ET_OK;
   for (it = 1; it <= maxiter; ++it) {
     ET_TRY(apply_A64(p_, q, OP_Y, nullptr));
+    prof.begin(ET_PROF_PCG_VEC, s);
     ET_TRY(dot(n_dof, p_, q, partial, sh.scal + SC_PQ, s));
     pcg_update_xr_kernel<<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, p_, q, x, r, partial);
     ET_CHECK_LAUNCH();
     ET_TRY(reduce_finalize(partial, sh.scal + SC_RR, s));
+    prof.end(s);
     ET_TRY(read_scalars());
     const double pq = host_scal[SC_PQ], rr = host_scal[SC_RR], bb = host_scal[SC_BB];
     R = std::sqrt(rr) / std::sqrt(bb);
@@ -541,10 +613,12 @@ int MGSolver::solve(const double* E, const uint8_t* cons, const double* b, doubl
     pcg_omega_kernel<<<1, 1, 0, s>>>(sh.scal, tol, min_omega, d_omega);
     ET_CHECK_LAUNCH();
     ET_TRY(prec());
+    prof.begin(ET_PROF_PCG_VEC, s);
     ET_TRY(dot(n_dof, r, z, partial, sh.scal + SC_RHO_NEW, s));
     pcg_update_p_kernel<<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, z, p_);
     ET_CHECK_LAUNCH();
     pcg_rotate_kernel<<<1, 1, 0, s>>>(sh.scal);
+    prof.end(s);
     ET_CHECK_LAUNCH();
   }
   if (it > maxiter) it = maxiter;

@@ -10,8 +10,55 @@ namespace et {
 // device scalar slots of the PCG loop
 enum { SC_BB = 0, SC_RHO, SC_RHO_NEW, SC_PQ, SC_RR, SC_OMEGA, SC_COUNT };
 
+// Event-based per-kernel-class timing (enabled on demand).
+struct Prof {
+  bool on = false;
+  struct Rec {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;  // recorded this window
+  std::vector<hipEvent_t> pool;
+  size_t next = 0;
+  int open_kind = -1;
+  hipEvent_t ev() {
+    if (next == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[next++];
+  }
+  void begin(int kind, hipStream_t s) {
+    if (!on || recs.size() > 200000) return;
+    Rec r{kind, ev(), ev()};
+    if (!r.a || !r.b) return;
+    (void)hipEventRecord(r.a, s);
+    recs.push_back(r);
+    open_kind = kind;
+  }
+  void end(hipStream_t s) {
+    if (!on || open_kind < 0) return;
+    (void)hipEventRecord(recs.back().b, s);
+    open_kind = -1;
+  }
+  void reset() {
+    recs.clear();
+    next = 0;
+    open_kind = -1;
+  }
+  void release() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+    recs.clear();
+    next = 0;
+  }
+};
+
 // operator data shared by all levels
 struct MGShared {
+  Prof* prof = nullptr;
+  bool tiled = false;  // fine grid uses the LDS-tiled H8 operator
   double* K0d = nullptr;
   float* K0f = nullptr;
   const double* Ed = nullptr;  // caller's E (FP64)
@@ -63,12 +110,18 @@ struct MGSolver {
   MGHierarchy<float> h32;
   CoarseSolver coarse;
   std::vector<double*> S64;  // FP64 Galerkin stencils, level 0..n_level
+  bool elem_galerkin = false;  // level 1 built from elements (level-0 stencil not formed)
+  double* Mt = nullptr;        // child tables W_t^T K0 W_t
+  uint8_t* eflag = nullptr;    // per-element "touches a constrained DOF"
+
   double *r = nullptr, *z = nullptr, *p_ = nullptr, *q = nullptr, *diag64 = nullptr;
   double* partial = nullptr;
   double* host_scal = nullptr;
   double max_omega = 0, min_omega = 0, d_omega = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   double t_setup_ms = 0, t_pcg_ms = 0;
+  Prof prof;
+  int profile_read(double* ms, int64_t* launches);
 
   int create(const et_mg_params& p, const double* K0_host, hipStream_t s);
   void destroy();

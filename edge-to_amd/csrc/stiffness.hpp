@@ -13,6 +13,17 @@ int matvec(const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T*
 template <typename T>
 int diagonal(const Grid& g, const T* K0, const T* E, const uint8_t* cons, T* diag, hipStream_t s);
 
+// LDS-tiled fine-grid H8 operator; mode: 0 y = Ax, 1 y = b - Ax,
+// 2 y = x + w D^-1 (b - Ax), 3 the same plus xadd.  Element index of (ex, ey, ez)
+// is ex*e_sx + ey*e_sy + ez.  omega/wscale only used by modes 2 and 3.
+template <typename T>
+int op0_tiled(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E,
+              const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
+              double wscale, const T* xadd, T* y, hipStream_t s);
+bool tiled_supported(const Grid& g, size_t elem_bytes);
+void set_tiled_jc(int jc);
+int get_tiled_jc();
+
 // mode 0: -u^T K0 u; mode 1: SIMP sensitivity p*clip(xt)^(p-1) * (-u^T K0 u)
 int elem_energy(const Grid& g, const double* K0, const double* U, const double* xt, double penal,
                 double vmin, int mode, double* out, hipStream_t s);

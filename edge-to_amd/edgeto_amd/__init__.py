@@ -25,6 +25,7 @@ from .physics import (auto_stiffness, hexahedron_element_stiffness,
                       quadrilateral_element_stiffness)
 from .solvers import CuMultiGrid, MultiGrid, Solver
 from .TopOpt import CuTopOpt, TopOpt
+from .autograd import ComplianceFunction, compliance, energy_compliance
 
 _lib.load(require_gpu=False)
 
@@ -33,4 +34,5 @@ __all__ = ["CuStructuredMesh2D", "CuStructuredMesh3D", "CuStructuredFilter2D",
            "CuTopOpt", "auto_stiffness", "StructuredMesh2D", "StructuredMesh3D",
            "StructuredFilter2D", "StructuredFilter3D", "StructuredStiffnessKernel", "MultiGrid",
            "TopOpt", "FilterKernel", "StiffnessKernel", "MaterialModel", "Solver",
-           "quadrilateral_element_stiffness", "hexahedron_element_stiffness"]
+           "quadrilateral_element_stiffness", "hexahedron_element_stiffness", "compliance",
+           "ComplianceFunction", "energy_compliance"]

@@ -40,6 +40,8 @@ _D = ctypes.c_double
 _IP = ctypes.POINTER(ctypes.c_int)
 
 _SIGS = {
+    "et_tune_set": [_I, _I],
+    "et_tune_get": [_I],
     "et_generate_elements": [_I, _IP, _P, _P],
     "et_matvec_f64": [_I, _IP, _P, _P, _P, _P, _P, _P],
     "et_matvec_f32": [_I, _IP, _P, _P, _P, _P, _P, _P],
@@ -55,6 +57,7 @@ _SIGS = {
     "et_stencil_from_operator_f64": [_I, _IP, _P, _P, _P, _P, _P],
     "et_galerkin_f64": [_I, _IP, _I, _P, _P, _P],
     "et_stencil_apply_f64": [_I, _IP, _I, _P, _P, _P, _P],
+    "et_galerkin_elements_f64": [_I, _IP, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P],
     "et_dot_f64": [_I64, _P, _P, _P, _P],
     "et_sqdist_f64": [_I64, _P, _P, _P, _P],
     "et_oc_work_size": [_I64],
@@ -63,7 +66,12 @@ _SIGS = {
     "et_mg_destroy": [_P],
     "et_mg_solve": [_P, _P, _P, _P, _P, _D, _I, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)],
     "et_mg_last_timings": [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+    "et_mg_profile": [_P, _I],
+    "et_mg_profile_read": [_P, _P, _P],
 }
+
+PROF_KINDS = ["l0_jacobi", "l0_residual", "l0_jacobi_add", "l0_scale", "matvec64", "coarse_ops",
+              "transfer", "coarse_solve", "pcg_vector", "setup"]
 
 EXPORTED = sorted(list(_SIGS) + ["et_last_error", "et_version"])
 

@@ -46,6 +46,12 @@ const char* et_last_error(void);
 /* Library version string. */
 const char* et_version(void);
 
+/* Launch-shape tuning knobs (process-wide; read by subsequent launches).
+ * ET_TUNE_TILED_JC: y planes marched per workgroup by the fine-grid operator. */
+enum { ET_TUNE_TILED_JC = 0 };
+int et_tune_set(int param, int value);
+int et_tune_get(int param);
+
 /* ---------------------------------------------------------------- mesh --- */
 
 /* Element connectivity (nel x 4|8, int32), bit-exact to
@@ -120,6 +126,12 @@ int et_stencil_from_operator_f64(int dim, const int* nel, const double* K0, cons
                                  const uint8_t* cons, double* S, hipStream_t stream);
 int et_galerkin_f64(int dim, const int* nel, int dof, const double* S_fine, double* S_coarse,
                     hipStream_t stream);
+/* First coarse operator R A_hat R^T built directly from the element densities
+ * (no fine-grid stencil), for even element counts.  K0_host is the element
+ * matrix in host memory.  Allocates and frees temporary device workspace and
+ * synchronises `stream`. */
+int et_galerkin_elements_f64(int dim, const int* nel, const double* K0_host, const double* E,
+                             const uint8_t* cons, double* S_coarse, hipStream_t stream);
 /* y = S x for a stencil operator on grid nel. */
 int et_stencil_apply_f64(int dim, const int* nel, int dof, const double* S, const double* x,
                          double* y, hipStream_t stream);
@@ -179,6 +191,26 @@ int et_mg_solve(et_mg* handle, const double* E, const uint8_t* cons, const doubl
 
 /* Per-phase timings of the last solve in milliseconds (setup, pcg), via HIP events. */
 int et_mg_last_timings(et_mg* handle, double* setup_ms, double* pcg_ms);
+
+/* Kernel-class profiling with HIP events recorded around each launch on the
+ * solver's stream.  et_mg_profile(h, 1) clears and enables, (h, 0) disables.
+ * et_mg_profile_read synchronises and fills ms[k] / launches[k] for the
+ * ET_PROF_KINDS classes below (accumulated since the last enable). */
+enum {
+  ET_PROF_L0_JAC = 0,      /* fine-grid damped-Jacobi sweep  y = x + w D^-1 (b - A x)   */
+  ET_PROF_L0_RES = 1,      /* fine-grid residual             y = b - A x               */
+  ET_PROF_L0_JACADD = 2,   /* last post-smoothing sweep fused with x += e             */
+  ET_PROF_L0_SCALE = 3,    /* first sweep from a zero guess  y = w D^-1 b              */
+  ET_PROF_MATVEC64 = 4,    /* outer FP64 operator (PCG q = A p, initial residual)     */
+  ET_PROF_COARSE_OPS = 5,  /* stencil sweeps / residuals on levels >= 1                */
+  ET_PROF_TRANSFER = 6,    /* restriction and prolongation                            */
+  ET_PROF_COARSE_SOLVE = 7,/* dense coarsest-grid solve                               */
+  ET_PROF_PCG_VEC = 8,     /* PCG dots, updates and precision conversions             */
+  ET_PROF_SETUP = 9,       /* hierarchy construction and coarse factorisation         */
+  ET_PROF_KINDS = 10
+};
+int et_mg_profile(et_mg* handle, int enable);
+int et_mg_profile_read(et_mg* handle, double* ms, int64_t* launches);
 
 #ifdef __cplusplus
 }

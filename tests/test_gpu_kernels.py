@@ -261,3 +261,32 @@ def test_oc_update(n, seed):
     L.call("et_oc_update_f64", n, L.ptr(dev(x)), L.ptr(dev(np.zeros(n))), 1.0, float(n), 0.4, 0.2,
            L.ptr(out), L.ptr(work), ctypes.byref(st), L.stream())
     assert rel(out.cpu().numpy(), ref0) < 1e-14
+
+
+@pytest.mark.parametrize("nel", [(8, 4), (12, 6), (6, 4, 4), (8, 4, 6), (4, 2, 2)])
+def test_galerkin_from_elements(nel):
+    """Element-based first coarse operator equals R A_hat R^T (with constraints)."""
+    L = _lib()
+    p = _problem(nel, seed=11, frac_cons=0.1)
+    dim, dof = p["dim"], p["dim"]
+    # clamp one face as well so whole elements carry constraints
+    nn = np.array(nel) + 1
+    cons = p["cons"].copy()
+    if dim == 3:
+        j, k = np.meshgrid(np.arange(nn[1]), np.arange(nn[2]), indexing="ij")
+        face = (j * nn[0] * nn[2] + k).ravel()
+    else:
+        face = np.arange(nn[1]) * nn[0]
+    for d in range(dof):
+        cons[face * dof + d] = True
+    NO = 27 if dim == 3 else 9
+    nc = tuple(v // 2 for v in nel)
+    Nc = int(np.prod(np.array(nc) + 1))
+    Sc = torch.empty(NO * dof * dof * Nc, dtype=torch.float64, device="cuda")
+    K0 = np.ascontiguousarray(p["K0"])
+    L.call("et_galerkin_elements_f64", dim, L.nel_arg(nel), K0.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+           L.ptr(dev(p["E"])), L.ptr(dev(cons.astype(np.uint8))), L.ptr(Sc), L.stream())
+    A = oops.assemble(p["K0"], p["el"], p["E"], cons, p["n_dof"])
+    R = omgm.restriction_matrix(nel, dof)
+    ref = (R @ A @ R.T).toarray()
+    assert rel(_stencil_to_dense(Sc.cpu().numpy(), nc, dof), ref) < 1e-12
