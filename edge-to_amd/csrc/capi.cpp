@@ -36,14 +36,28 @@ const char* et_version(void) { return "edgeto_amd 0.1 (gfx950)"; }
 
 int et_tune_set(int param, int value) {
   ET_GUARD({
-    ET_REQUIRE(param == ET_TUNE_TILED_JC, "et_tune_set: unknown parameter");
-    ET_REQUIRE(value >= 1 && value <= 1024, "et_tune_set: value out of range");
-    et::set_tiled_jc(value);
+    ET_REQUIRE(value >= 0 && value <= 1024, "et_tune_set: value out of range");
+    if (param == ET_TUNE_TILED_JC) {
+      ET_REQUIRE(value >= 1, "et_tune_set: jc must be >= 1");
+      et::set_tiled_jc(value);
+    } else if (param == ET_TUNE_MFMA_JC) {
+      ET_REQUIRE(value >= 1, "et_tune_set: jc must be >= 1");
+      et::set_mfma_jc(value);
+    } else if (param == ET_TUNE_MIXED_MFMA) {
+      et::set_mixed_mfma(value);
+    } else {
+      ET_REQUIRE(false, "et_tune_set: unknown parameter");
+    }
     return ET_OK;
   })
 }
 
-int et_tune_get(int param) { return param == ET_TUNE_TILED_JC ? et::get_tiled_jc() : -1; }
+int et_tune_get(int param) {
+  if (param == ET_TUNE_TILED_JC) return et::get_tiled_jc();
+  if (param == ET_TUNE_MFMA_JC) return et::get_mfma_jc();
+  if (param == ET_TUNE_MIXED_MFMA) return et::get_mixed_mfma();
+  return -1;
+}
 
 int et_generate_elements(int dim, const int* nel, int32_t* elements, hipStream_t stream) {
   ET_GUARD({
@@ -73,6 +87,20 @@ int et_matvec_f32(int dim, const int* nel, const float* K0, const float* E, cons
     ET_REQUIRE(K0 && E && u && y, "et_matvec_f32: null pointer");
     ET_REQUIRE(u != y, "et_matvec_f32: u and y must not alias");
     return et::matvec<float>(g, K0, E, cons, u, y, stream);
+  })
+}
+
+int et_matvec_f32_mfma(int dim, const int* nel, const double* K0_host, const float* K0,
+                       const float* E, const uint8_t* cons, const float* u, float* y,
+                       hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(K0_host && K0 && E && u && y, "et_matvec_f32_mfma: null pointer");
+    ET_REQUIRE(u != y, "et_matvec_f32_mfma: u and y must not alias");
+    ET_REQUIRE(et::mfma_supported(g), "et_matvec_f32_mfma: 3-D grids only (nz <= ~290)");
+    return et::op0_mfma(0, g, (int64_t)g.ey * g.ez, g.ez, et::k0_scale(K0_host, 576), K0, E, cons, u,
+                        nullptr, nullptr, nullptr, 1.0, nullptr, y, stream);
   })
 }
 

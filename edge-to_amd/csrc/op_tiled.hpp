@@ -2,11 +2,14 @@
 //
 // A workgroup owns a strip of 256 consecutive nodes of one x-z node plane
 // (plane index p = i*nz + k, contiguous in memory) and marches over JC
-// consecutive y planes.  Per plane it stages in LDS: the u values of the strip
-// plus a halo of nz+1 nodes on each side (SoA per component, constrained
-// entries zeroed), in a ring of three planes, and the E values of the element
-// columns the strip touches, in a ring of two element planes.  Each thread
-// then forms
+// consecutive y planes.  LDS holds a ring of four u planes (the strip plus a
+// halo of nz+1 nodes on each side, SoA per component, constrained entries
+// zeroed) and a ring of three E planes (the element columns the strip
+// touches).  The loop is software-pipelined: while plane j is computed, the
+// global loads of u plane j+2, E plane j+1 and of this plane's epilogue
+// operands are in flight in registers; they are written to LDS after the
+// compute, so one barrier per plane suffices and memory latency overlaps the
+// FMAs.  Each thread forms
 //     y_n = sum_a E_a * sum_o K0[a, a+o] u_{n+o}
 // visiting the 27 neighbours o once (81 LDS reads) and keeping one partial
 // 3-vector per adjacent element a.  Every output is written once: no
@@ -18,19 +21,21 @@
 // components:  K0[a, b] = R_a K0[0, a^b] R_a  (corner bits XORed, R_a the
 // diagonal sign matrix).  Only the first block row of K0 (8 blocks of 3x3,
 // 72 values) is therefore needed; it is read with scalar loads and the signs
-// are compile-time constants folded into the FMAs.  For a fixed neighbour offset
-// o every pair (a, a+o) uses the same base block, the one whose XOR pattern
-// marks the non-zero components of o.
+// are compile-time constants folded into the FMAs.  For a fixed neighbour
+// offset o every pair (a, a+o) uses the same base block, the one whose XOR
+// pattern marks the non-zero components of o.
 #pragma once
 #include "common.hpp"
 
 namespace et {
 
 constexpr int kStrip = 256;
+constexpr int kUSlots = 3;  // u nodes staged per thread per plane (u_span <= 768)
+constexpr int kESlots = 4;  // E values staged per thread per plane (e_span <= 1024)
 
 // Base-block entry (r, block m, c) = K0[r][3m + c]: wave-uniform, compile-time
 // offsets, so it is fetched with scalar loads into SGPRs.
-#define ET_TILED_K(r, m, c) K0[(r) * 24 + 3 * (m) + (c)]
+#define ET_TILED_K(r, m, c) Kp[(r) * 24 + 3 * (m) + (c)]
 
 enum TiledMode { TM_Y = 0, TM_RES = 1, TM_JAC = 2, TM_JAC_ADD = 3 };
 
@@ -57,9 +62,13 @@ __host__ inline TiledArgs tiled_args(const Grid& g, int64_t e_sx, int64_t e_sy, 
   return a;
 }
 
+__host__ inline bool tiled_shape_ok(const TiledArgs& a) {
+  return a.g.dim == 3 && a.u_span <= kUSlots * kStrip && a.e_span <= kESlots * kStrip;
+}
+
 template <typename T>
 __host__ inline size_t tiled_lds_bytes(const TiledArgs& a) {
-  return (size_t)3 * 3 * a.u_span * sizeof(T) + (size_t)2 * a.e_span * sizeof(T);
+  return (size_t)4 * 3 * a.u_span * sizeof(T) + (size_t)3 * a.e_span * sizeof(T);
 }
 
 // offset bit of H8 corner a along axis d (0 = x, 1 = y, 2 = z)
@@ -81,54 +90,127 @@ __global__ __launch_bounds__(256) void op0_tiled_kernel(const TiledArgs A, const
                                                         double wscale, const T* __restrict__ xadd,
                                                         T* __restrict__ y) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  T* su = reinterpret_cast<T*>(lds_raw);        // [3 planes][3 comps][u_span]
-  T* se = su + 9 * A.u_span;                    // [2 planes][e_span]
+  T* su = reinterpret_cast<T*>(lds_raw);  // [4 planes][3 comps][u_span]
+  T* se = su + 12 * A.u_span;             // [3 planes][e_span]
   const Grid& g = A.g;
   const int nx = g.nx(), ny = g.ny(), nz = g.nz();
-  const int64_t plane = (int64_t)nx * nz;
-  const int64_t p0 = (int64_t)blockIdx.x * kStrip;
+  const int plane = nx * nz;
+  const int p0 = blockIdx.x * kStrip;
   const int j0 = blockIdx.y * A.jc;
   const int j1 = min(j0 + A.jc, ny);
   const int tid = threadIdx.x;
-  const int64_t p = p0 + tid;
+  const int p = p0 + tid;
   const bool active = p < plane;
-  const int i = (int)(p / nz), k = (int)(p % nz);
-  const int ex0 = (int)(p0 / nz) - 1;  // first element column staged
-  const int64_t ubase = p0 - A.halo;   // plane index of LDS slot 0
+  const int i = p / nz, k = p - (p / nz) * nz;
+  const int ex0 = p0 / nz - 1;    // first element column staged
+  const int ubase = p0 - A.halo;  // plane index of LDS slot 0
 
-  auto load_u = [&](int jp) {
-    T* dst = su + (size_t)((jp + 3) % 3) * 3 * A.u_span;
+  // per-thread fixed staging slots
+  int uq[kUSlots];  // plane index of the u node staged by this thread
+  bool uin[kUSlots];
+#pragma unroll
+  for (int m = 0; m < kUSlots; ++m) {
+    const int s = tid + m * kStrip;
+    const int q = ubase + s;
+    uq[m] = q;
+    uin[m] = s < A.u_span && q >= 0 && q < plane;
+  }
+  int eoff[kESlots];  // ex*e_sx + ez (fits 32 bits: < n_elem)
+  bool ein[kESlots];
+#pragma unroll
+  for (int m = 0; m < kESlots; ++m) {
+    const int t = tid + m * kStrip;
+    const int col = t / g.ez, ez = t - col * g.ez;
+    const int ex = ex0 + col;
+    ein[m] = t < A.e_span && ex >= 0 && ex < g.ex;
+    eoff[m] = ein[m] ? (int)((int64_t)ex * A.e_sx + ez) : 0;
+  }
+
+  T pu[kUSlots][3];
+  T pe[kESlots];
+  auto fetch_u = [&](int jp) {
     const bool in = jp >= 0 && jp < ny;
-    const int64_t nvals = (int64_t)A.u_span * 3;
-    for (int64_t t = tid; t < nvals; t += blockDim.x) {
-      const int64_t q = ubase + t / 3;
-      const int c = (int)(t % 3);
-      T v = T(0);
-      if (in && q >= 0 && q < plane) {
-        const int64_t d = ((int64_t)jp * plane + q) * 3 + c;
-        v = (cons != nullptr && cons[d]) ? T(0) : x[d];
+    // plane base pointers are wave-uniform; per-thread offsets stay 32-bit
+    const int64_t pb = (int64_t)(in ? jp : 0) * plane * 3;
+    const T* xp = x + pb;
+    const uint8_t* cp = cons != nullptr ? cons + pb : nullptr;
+#pragma unroll
+    for (int m = 0; m < kUSlots; ++m) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        T v = T(0);
+        if (in && uin[m]) {
+          const int d = uq[m] * 3 + c;
+          v = xp[d];
+          if (cp != nullptr && cp[d]) v = T(0);
+        }
+        pu[m][c] = v;
       }
-      dst[c * A.u_span + t / 3] = v;
     }
   };
-  auto load_e = [&](int ey) {
-    T* dst = se + (size_t)((ey + 2) % 2) * A.e_span;
+  auto store_u = [&](int jp) {
+    T* dst = su + (size_t)((jp + 4) & 3) * 3 * A.u_span;
+#pragma unroll
+    for (int m = 0; m < kUSlots; ++m) {
+      const int s = tid + m * kStrip;
+      if (s < A.u_span) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dst[c * A.u_span + s] = pu[m][c];
+      }
+    }
+  };
+  auto fetch_e = [&](int ey) {
     const bool in = ey >= 0 && ey < g.ey;
-    for (int t = tid; t < A.e_span; t += blockDim.x) {
-      const int ex = ex0 + t / g.ez, ez = t % g.ez;
-      T v = T(0);
-      if (in && ex >= 0 && ex < g.ex) v = E[(int64_t)ex * A.e_sx + (int64_t)ey * A.e_sy + ez];
-      dst[t] = v;
+    const T* ep = E + (int64_t)(in ? ey : 0) * A.e_sy;
+#pragma unroll
+    for (int m = 0; m < kESlots; ++m) pe[m] = (in && ein[m]) ? ep[eoff[m]] : T(0);
+  };
+  auto store_e = [&](int ey) {
+    T* dst = se + (size_t)((ey + 3) % 3) * A.e_span;
+#pragma unroll
+    for (int m = 0; m < kESlots; ++m) {
+      const int t = tid + m * kStrip;
+      if (t < A.e_span) dst[t] = pe[m];
     }
   };
 
-  load_u(j0 - 1);
-  load_u(j0);
-  load_e(j0 - 1);
+  // prologue: u planes j0-1, j0, j0+1 and E planes j0-1, j0
+  fetch_u(j0 - 1);
+  store_u(j0 - 1);
+  fetch_u(j0);
+  store_u(j0);
+  fetch_u(j0 + 1);
+  store_u(j0 + 1);
+  fetch_e(j0 - 1);
+  store_e(j0 - 1);
+  fetch_e(j0);
+  store_e(j0);
+  __syncthreads();
+
+  const T w = MODE >= TM_JAC ? (T)(*omega * wscale) : T(0);
   for (int j = j0; j < j1; ++j) {
-    load_u(j + 1);
-    load_e(j);
-    __syncthreads();
+    const bool more = j + 1 < j1;
+    if (more) {
+      fetch_u(j + 2);
+      fetch_e(j + 1);
+    }
+    // epilogue operands of this plane, issued before the FMAs
+    const int64_t jb = (int64_t)j * plane * 3;  // wave-uniform plane offset
+    const int pd = p * 3;                        // per-thread offset in the plane
+    T bv[3] = {T(0), T(0), T(0)}, dv[3] = {T(0), T(0), T(0)};
+    uint8_t cv[3] = {0, 0, 0};
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        if (cons != nullptr) cv[r] = cons[jb + pd + r];
+        if (MODE != TM_Y) bv[r] = b[jb + pd + r];
+        if (MODE >= TM_JAC) dv[r] = Dinv[jb + pd + r];
+      }
+    }
+    // Re-read K0 from memory in every plane: keeps the 72 scalar-loaded values
+    // from being hoisted out of the plane loop (they would not fit in SGPRs).
+    const T* Kp = K0;
+    asm volatile("" : "+s"(Kp));
     if (active) {
       T s[8][3];
 #pragma unroll
@@ -153,7 +235,7 @@ __global__ __launch_bounds__(256) void op0_tiled_kernel(const TiledArgs A, const
           const int di = ax ? (sx ? 1 : -1) : 0;
           const int dj = ay ? (sy ? 1 : -1) : 0;
           const int dk = az ? (sz ? 1 : -1) : 0;
-          const T* pl = su + (size_t)((j + dj + 3) % 3) * 3 * A.u_span;
+          const T* pl = su + (size_t)((j + dj + 4) & 3) * 3 * A.u_span;
           const int q = lp + di * nz + dk;
           // At k = 0 / k = nz-1 the slot q +- 1 wraps into the neighbouring x column;
           // such values only feed elements outside the grid, whose E is zero.
@@ -178,27 +260,31 @@ __global__ __launch_bounds__(256) void op0_tiled_kernel(const TiledArgs A, const
       for (int a = 0; a < 8; ++a) {
         const int ex = i - hx(a), ey = j - hy(a), ez = k - hz(a);
         const bool zin = ez >= 0 && ez < g.ez;
-        const T Ea = se[(size_t)((ey + 2) % 2) * A.e_span + (ex - ex0) * g.ez + (zin ? ez : 0)];
+        const T Ea = se[(size_t)((ey + 3) % 3) * A.e_span + (ex - ex0) * g.ez + (zin ? ez : 0)];
 #pragma unroll
         for (int r = 0; r < 3; ++r) acc[r] += (zin ? Ea : T(0)) * s[a][r];
       }
-      const int64_t n = (int64_t)j * plane + p;
-      const T w = MODE >= TM_JAC ? (T)(*omega * wscale) : T(0);
+      const T* own = su + (size_t)((j + 4) & 3) * 3 * A.u_span;
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
-        const int64_t d = n * 3 + r;
-        const T xv = x[d];
-        const T ax = (cons != nullptr && cons[d]) ? xv : acc[r];
+        const int64_t d = jb + pd + r;
+        T xv = own[r * A.u_span + lp];  // zero on constrained DOFs
+        if (cv[r]) xv = x[d];
+        const T ax = cv[r] ? xv : acc[r];
         T v;
         if (MODE == TM_Y)
           v = ax;
         else if (MODE == TM_RES)
-          v = b[d] - ax;
+          v = bv[r] - ax;
         else
-          v = xv + w * Dinv[d] * (b[d] - ax);
+          v = xv + w * dv[r] * (bv[r] - ax);
         if (MODE == TM_JAC_ADD) v += xadd[d];
         y[d] = v;
       }
+    }
+    if (more) {
+      store_u(j + 2);
+      store_e(j + 1);
     }
     __syncthreads();
   }

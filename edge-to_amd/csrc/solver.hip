@@ -234,6 +234,13 @@ static int op_launch(const MGLevel<T>& L, const MGShared& sh, int mode, const T*
                      const double* omega, double wscale, const T* xadd, T* y, hipStream_t s) {
   const Grid& g = L.g;
   sh.prof->begin(op_kind(L.level, mode), s);
+  if (L.level == 0 && sizeof(T) == 4 && sh.mfma) {
+    const int st = op0_mfma(mode, g, (int64_t)g.ey * g.ez, g.ez, sh.kscale, sh.K0f, sh.Ef, sh.cons,
+                            (const float*)x, (const float*)b, (const float*)L.Dinv, omega, wscale,
+                            (const float*)xadd, (float*)y, s);
+    sh.prof->end(s);
+    return st;
+  }
   if (L.level == 0 && sh.tiled) {
     const int st = op0_tiled<T>(mode, g, (int64_t)g.ey * g.ez, g.ez, sh.K0<T>(), sh.E<T>(), sh.cons,
                                 x, b, L.Dinv, omega, wscale, xadd, y, s);
@@ -398,6 +405,8 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
   n_dof = g0.n_nodes() * dof;
   n_elem = g0.n_elem();
   sh.tiled = g0.dim == 3 && tiled_supported(g0, sizeof(double));
+  sh.mfma = p.precision == 1 && get_mixed_mfma() && mfma_supported(g0);
+  sh.kscale = k0_scale(K0_host, (p.dim == 3 ? 24 * 24 : 8 * 8));
   const int nd = (p.dim == 3 ? 8 : 4) * dof;
   ET_CHECK_HIP(hipMalloc(&sh.K0d, nd * nd * sizeof(double)));
   ET_CHECK_HIP(hipMalloc(&sh.K0f, nd * nd * sizeof(float)));

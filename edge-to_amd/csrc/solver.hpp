@@ -59,6 +59,8 @@ struct Prof {
 struct MGShared {
   Prof* prof = nullptr;
   bool tiled = false;  // fine grid uses the LDS-tiled H8 operator
+  bool mfma = false;   // FP32 fine-grid sweeps use the matrix-core operator
+  float kscale = 1.f;  // power-of-two scale of K0 for the FP16 split
   double* K0d = nullptr;
   float* K0f = nullptr;
   const double* Ed = nullptr;  // caller's E (FP64)

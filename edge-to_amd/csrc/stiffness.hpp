@@ -21,6 +21,16 @@ int op0_tiled(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, 
               const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
               double wscale, const T* xadd, T* y, hipStream_t s);
 bool tiled_supported(const Grid& g, size_t elem_bytes);
+// Matrix-core FP32 variant (FP16 hi/lo MFMA); kscale: power of two >= max|K0|.
+int op0_mfma(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, float kscale, const float* K0,
+             const float* E, const uint8_t* cons, const float* x, const float* b, const float* Dinv,
+             const double* omega, double wscale, const float* xadd, float* y, hipStream_t s);
+bool mfma_supported(const Grid& g);
+void set_mfma_jc(int jc);
+int get_mfma_jc();
+void set_mixed_mfma(int on);
+int get_mixed_mfma();
+float k0_scale(const double* K0_host, int n);
 void set_tiled_jc(int jc);
 int get_tiled_jc();
 

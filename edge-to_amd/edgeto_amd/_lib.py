@@ -45,6 +45,7 @@ _SIGS = {
     "et_generate_elements": [_I, _IP, _P, _P],
     "et_matvec_f64": [_I, _IP, _P, _P, _P, _P, _P, _P],
     "et_matvec_f32": [_I, _IP, _P, _P, _P, _P, _P, _P],
+    "et_matvec_f32_mfma": [_I, _IP, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P, _P, _P],
     "et_diagonal_f64": [_I, _IP, _P, _P, _P, _P, _P],
     "et_elem_energy_f64": [_I, _IP, _P, _P, _P, _P],
     "et_sensitivity_f64": [_I, _IP, _P, _P, _P, _D, _D, _P, _P],

@@ -47,8 +47,11 @@ const char* et_last_error(void);
 const char* et_version(void);
 
 /* Launch-shape tuning knobs (process-wide; read by subsequent launches).
- * ET_TUNE_TILED_JC: y planes marched per workgroup by the fine-grid operator. */
-enum { ET_TUNE_TILED_JC = 0 };
+ * ET_TUNE_TILED_JC: y planes marched per workgroup by the fine-grid operator.
+ * ET_TUNE_MFMA_JC: the same for the matrix-core FP32 operator.
+ * ET_TUNE_MIXED_MFMA: 1 = mixed-precision V-cycles use the matrix-core fine-grid
+ * operator; 0 (default) = FP32 vector FMAs.  Read by et_mg_create. */
+enum { ET_TUNE_TILED_JC = 0, ET_TUNE_MFMA_JC = 1, ET_TUNE_MIXED_MFMA = 2 };
 int et_tune_set(int param, int value);
 int et_tune_get(int param);
 
@@ -68,6 +71,13 @@ int et_matvec_f64(int dim, const int* nel, const double* K0, const double* E,
                   const uint8_t* cons, const double* u, double* y, hipStream_t stream);
 int et_matvec_f32(int dim, const int* nel, const float* K0, const float* E,
                   const uint8_t* cons, const float* u, float* y, hipStream_t stream);
+
+/* The same operator for 3-D grids on the matrix cores: FP32 vectors, element
+ * contraction in FP16 MFMA with a hi/lo split of both operands (~22 bits),
+ * FP32 accumulation.  K0_host (host, FP64) gives the power-of-two K0 scale. */
+int et_matvec_f32_mfma(int dim, const int* nel, const double* K0_host, const float* K0,
+                       const float* E, const uint8_t* cons, const float* u, float* y,
+                       hipStream_t stream);
 
 /* diag(A_hat); constrained entries are 1.  Replaces
  * get_diagonal_node_basis_cuda_kernel (core/_templated_cuda.cu:1068-1113). */

@@ -290,3 +290,19 @@ def test_galerkin_from_elements(nel):
     R = omgm.restriction_matrix(nel, dof)
     ref = (R @ A @ R.T).toarray()
     assert rel(_stencil_to_dense(Sc.cpu().numpy(), nc, dof), ref) < 1e-12
+
+
+@pytest.mark.parametrize("nel,scale", [((6, 4, 4), 1.0), ((8, 6, 20), 1e-7), ((5, 3, 33), 3e4), ((12, 5, 17), 1.0)])
+def test_matvec_f32_mfma(nel, scale):
+    """Matrix-core fine operator (FP16 hi/lo split, per-element scaling) at FP32-level accuracy,
+    including tiny and large vector magnitudes and z extents that are not multiples of 16."""
+    L = _lib()
+    p = _problem(nel, seed=12)
+    u = p["u"] * scale
+    ref = oops.matvec(p["K0"], p["el"], p["E"], u, p["cons"])
+    y = torch.empty(p["n_dof"], dtype=torch.float32, device="cuda")
+    K0 = np.ascontiguousarray(p["K0"])
+    L.call("et_matvec_f32_mfma", 3, L.nel_arg(nel), K0.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+           L.ptr(dev(p["K0"], torch.float32)), L.ptr(dev(p["E"], torch.float32)),
+           L.ptr(dev(p["cons"].astype(np.uint8))), L.ptr(dev(u, torch.float32)), L.ptr(y), L.stream())
+    assert rel(y.cpu().numpy(), ref) < 2e-5
