@@ -15,7 +15,8 @@ mixed leg, timed with HIP events on the solver stream over the timed region.
 `cpu_baseline` times the NumPy/SciPy oracle on a bounded sample (see
 `cpu_baseline.sample`).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c5]
+(c5: 64 seeded random 2-D/3-D problems solved in turn; `steps` OC iterations each.)
 """
 import argparse
 import ctypes
@@ -93,6 +94,89 @@ def run_leg(ET, wl, precision, warmup, steps, profile=False):
                 n_dof=tp.kernel.shape[0], n_el=mesh.n_el)
 
 
+def batch_problem(i):
+    """Problem i of the 64-problem batch: seeded by default_rng(1000 + i)."""
+    rng = np.random.default_rng(1000 + i)
+    dim = 2 if rng.random() < 0.5 else 3
+    if dim == 2:
+        nel = (int(rng.integers(64, 513)), int(rng.integers(32, 257)))
+    else:
+        nx = int(rng.integers(32, 97)) // 2 * 2
+        nel = (nx, nx // 2, nx // 2)
+    side = int(rng.integers(0, 4))  # clamped side: x = 0, x = L, y = 0, y = H
+    n_loads = int(rng.integers(1, 5))
+    vf = float(rng.uniform(0.2, 0.6))
+    rmin = float(rng.uniform(1.2, 3.0))
+    nn = [n + 1 for n in nel]
+    idx = np.indices(nn[::-1] if dim == 2 else (nn[1], nn[0], nn[2]))
+    if dim == 2:
+        j, i_ = idx[0].ravel(), idx[1].ravel()
+        k = np.zeros_like(i_)
+        node = j * nn[0] + i_
+    else:
+        j, i_, k = idx[0].ravel(), idx[1].ravel(), idx[2].ravel()
+        node = (j * nn[0] + i_) * nn[2] + k
+    on_side = [i_ == 0, i_ == nel[0], j == 0, j == nel[1]]
+    boundary = on_side[0] | on_side[1] | on_side[2] | on_side[3]
+    if dim == 3:
+        boundary |= (k == 0) | (k == nel[2])
+    clamp = node[on_side[side]]
+    cand = node[boundary & ~on_side[side]]
+    loads = rng.choice(cand, size=n_loads, replace=False)
+    dirs = rng.standard_normal((n_loads, dim))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    n_level = 3
+    return dict(dim=dim, nel=nel, clamp=clamp, loads=loads, dirs=dirs, vf=vf, rmin=rmin, n_level=n_level)
+
+
+def run_batch(ET, n_problems, iters):
+    """Batched workload: independent random problems solved one after another on
+    one GPU (mixed precision); aggregate OC iterations/s and per-problem latency."""
+    import torch
+    lat, total_it, t_all = [], 0, 0.0
+    for i in range(n_problems):
+        pb = batch_problem(i)
+        nel = pb["nel"]
+        stiff = lambda x0: ET.auto_stiffness(x0, 1.0, 0.3)
+        if pb["dim"] == 2:
+            mesh = ET.CuStructuredMesh2D(*nel, *nel, stiffness=stiff)
+            filt = ET.CuStructuredFilter2D(mesh, pb["rmin"])
+        else:
+            mesh = ET.CuStructuredMesh3D(*nel, *nel, stiffness=stiff)
+            filt = ET.CuStructuredFilter3D(mesh, pb["rmin"])
+        kernel = ET.CuStructuredStiffnessKernel(mesh)
+        mat = ET.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=pb["vf"])
+        solver = ET.CuMultiGrid(mesh, kernel, tol=1e-8, n_smooth=2, n_level=pb["n_level"], precision="mixed")
+        tp = ET.CuTopOpt(mesh, mat, kernel, solver, filt, max_iter=0, ch_tol=0, fun_tol=0, verbose=False)
+        tp.add_BC_nodal(pb["clamp"], np.ones((1, pb["dim"]), dtype=np.int64))
+        tp.add_F_nodal(pb["loads"], pb["dirs"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = tp.material_model(torch.full((mesh.n_el,), pb["vf"], dtype=torch.float64, device="cuda"), 0,
+                              plain=True)
+        tp.kernel.set_rho(x)
+        comp, df, _ = tp.solve_and_process(x, 0)
+        for it in range(iters):
+            x = tp.update_design(x, df)
+            comp, df, _ = tp.solve_and_process(x, it)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        lat.append(dt / iters)
+        total_it += iters
+        t_all += dt
+        print(f"[bench] batch problem {i}: {pb['dim']}-D {'x'.join(map(str, nel))}, "
+              f"{dt / iters * 1e3:.1f} ms/iter, comp {comp:.6e}", file=sys.stderr, flush=True)
+    lat = np.array(lat)
+    return {"metric": "SIMP iters/s, 64-problem batch (mixed GMG-PCG)", "value": round(total_it / t_all, 3),
+            "unit": "SIMP iters/s", "n_gpus": 1, "steps": iters, "warmup": 0,
+            "ms_per_step": round(t_all / total_it * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 V-cycle + f64 PCG", "data": "synthetic (seeded random problems)",
+            "config": {"workload": f"{n_problems} random 2-D/3-D cantilever problems, default_rng(1000 + i), "
+                                   f"initial solve + {iters} OC iterations each, 4-grid GMG", "parallelism": "single GPU"},
+            "latency_ms_per_iter": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                                    "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)}}
+
+
 def roofline(prof, n_dof, n_el):
     """Dominant kernel class of the mixed leg and its achieved algorithmic bandwidth."""
     s = 4  # FP32 V-cycle vectors / E
@@ -114,7 +198,7 @@ def roofline(prof, n_dof, n_el):
             "bytes_per_launch": bytes_per[name]}
 
 
-def cpu_baseline(nel_metric, sample_nel=(48, 24, 24), n_level=2):
+def cpu_baseline(nel_metric, sample_nel=(96, 48, 48), n_level=3):
     """Oracle (NumPy/SciPy, FP64, one thread) on a bounded sample, scaled by element count."""
     from threadpoolctl import threadpool_limits
     sys.path.insert(0, ROOT)
@@ -152,16 +236,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c5"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--legs", default="mixed,fp64")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); the multi-GPU path is "
-              "a design (see DESIGN.md) and is not built.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); no multi-GPU path "
+              "is built (the slab sharding along y is a design only).", file=sys.stderr)
         sys.exit(2)
     import torch
     import edgeto_amd as ET
+    if args.workload == "c5":
+        print(json.dumps(run_batch(ET, 64, args.steps)))
+        return
     wl = WORKLOADS[args.workload]
     legs = {}
     for leg in args.legs.split(","):

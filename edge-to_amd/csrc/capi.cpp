@@ -1,6 +1,7 @@
 // extern "C" entry points declared in include/edgeto_amd.h.  Each one
 // validates its arguments, converts them to the internal Grid description and
 // forwards to the kernels; no exception crosses this boundary.
+#include <cmath>
 #include <new>
 #include <string>
 
@@ -29,6 +30,32 @@ using et::Grid;
     return ET_EINVAL;                                    \
   }
 
+namespace {
+
+template <typename T>
+static int residual_impl(const char* name, int dim, const int* nel, const T* K0, const T* E,
+                         const uint8_t* cons, const T* x, const T* b, T* r, hipStream_t stream) {
+  et::Grid g;
+  ET_TRY(et::make_grid(dim, nel, &g));
+  ET_REQUIRE(K0 && E && x && b && r, name);
+  ET_REQUIRE(r != x && r != b, "residual: r must not alias x or b");
+  return et::level0_op<T>(1, g, K0, E, cons, x, b, nullptr, nullptr, 1.0, nullptr, r, stream);
+}
+
+template <typename T>
+static int jacobi_impl(const char* name, int dim, const int* nel, const T* K0, const T* E,
+                       const uint8_t* cons, const T* x, const T* b, const T* Dinv, double omega, T* y,
+                       hipStream_t stream) {
+  et::Grid g;
+  ET_TRY(et::make_grid(dim, nel, &g));
+  ET_REQUIRE(K0 && E && x && b && Dinv && y, name);
+  ET_REQUIRE(y != x && y != b && y != Dinv, "jacobi: y must not alias x, b or Dinv");
+  ET_REQUIRE(std::isfinite(omega), "jacobi: omega must be finite");
+  return et::level0_op<T>(2, g, K0, E, cons, x, b, Dinv, nullptr, omega, nullptr, y, stream);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* et_last_error(void) { return et::last_error(); }
@@ -45,6 +72,16 @@ int et_tune_set(int param, int value) {
       et::set_mfma_jc(value);
     } else if (param == ET_TUNE_MIXED_MFMA) {
       et::set_mixed_mfma(value);
+    } else if (param == ET_TUNE_MX_F32 || param == ET_TUNE_MX_F64) {
+      et::set_mx(param == ET_TUNE_MX_F64, value);
+    } else if (param == ET_TUNE_MX_WGS_F32 || param == ET_TUNE_MX_WGS_F64) {
+      ET_REQUIRE(value >= 1 && value <= 8, "et_tune_set: workgroups per CU must be in [1, 8]");
+      et::set_mx_wgs(param == ET_TUNE_MX_WGS_F64, value);
+    } else if (param == ET_TUNE_WHT) {
+      et::set_wht(value);
+    } else if (param == ET_TUNE_WHT_WGS) {
+      ET_REQUIRE(value >= 1 && value <= 4, "et_tune_set: workgroups per CU must be in [1, 4]");
+      et::set_wht_wgs(value);
     } else {
       ET_REQUIRE(false, "et_tune_set: unknown parameter");
     }
@@ -56,6 +93,11 @@ int et_tune_get(int param) {
   if (param == ET_TUNE_TILED_JC) return et::get_tiled_jc();
   if (param == ET_TUNE_MFMA_JC) return et::get_mfma_jc();
   if (param == ET_TUNE_MIXED_MFMA) return et::get_mixed_mfma();
+  if (param == ET_TUNE_MX_F32 || param == ET_TUNE_MX_F64) return et::get_mx(param == ET_TUNE_MX_F64);
+  if (param == ET_TUNE_MX_WGS_F32 || param == ET_TUNE_MX_WGS_F64)
+    return et::get_mx_wgs(param == ET_TUNE_MX_WGS_F64);
+  if (param == ET_TUNE_WHT) return et::get_wht();
+  if (param == ET_TUNE_WHT_WGS) return et::get_wht_wgs();
   return -1;
 }
 
@@ -87,6 +129,29 @@ int et_matvec_f32(int dim, const int* nel, const float* K0, const float* E, cons
     ET_REQUIRE(K0 && E && u && y, "et_matvec_f32: null pointer");
     ET_REQUIRE(u != y, "et_matvec_f32: u and y must not alias");
     return et::matvec<float>(g, K0, E, cons, u, y, stream);
+  })
+}
+
+int et_residual_f64(int dim, const int* nel, const double* K0, const double* E, const uint8_t* cons,
+                    const double* x, const double* b, double* r, hipStream_t stream) {
+  ET_GUARD({ return residual_impl<double>("et_residual_f64: null pointer", dim, nel, K0, E, cons, x, b, r, stream); })
+}
+int et_residual_f32(int dim, const int* nel, const float* K0, const float* E, const uint8_t* cons,
+                    const float* x, const float* b, float* r, hipStream_t stream) {
+  ET_GUARD({ return residual_impl<float>("et_residual_f32: null pointer", dim, nel, K0, E, cons, x, b, r, stream); })
+}
+int et_jacobi_f64(int dim, const int* nel, const double* K0, const double* E, const uint8_t* cons,
+                  const double* x, const double* b, const double* Dinv, double omega, double* y,
+                  hipStream_t stream) {
+  ET_GUARD({
+    return jacobi_impl<double>("et_jacobi_f64: null pointer", dim, nel, K0, E, cons, x, b, Dinv, omega, y, stream);
+  })
+}
+int et_jacobi_f32(int dim, const int* nel, const float* K0, const float* E, const uint8_t* cons,
+                  const float* x, const float* b, const float* Dinv, double omega, float* y,
+                  hipStream_t stream) {
+  ET_GUARD({
+    return jacobi_impl<float>("et_jacobi_f32: null pointer", dim, nel, K0, E, cons, x, b, Dinv, omega, y, stream);
   })
 }
 

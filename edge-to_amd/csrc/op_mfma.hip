@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void op0_mfma_kernel(const MfmaArgs A, const f
     // its accumulator is only cleared)
     {
       float4_t* acc = sa + (size_t)(ey & 1) * kMfBX * nz;
-      const float w = MODE >= TM_JAC ? (float)(*omega * wscale) : 0.f;
+      const float w = MODE >= TM_JAC ? (float)((omega != nullptr ? *omega : 1.0) * wscale) : 0.f;
       const bool out = ey >= j0;
       for (int t = tid; t < kMfBX * nz; t += blockDim.x) {
         const int ncol = t / nz, k = t - ncol * nz;

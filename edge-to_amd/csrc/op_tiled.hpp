@@ -35,7 +35,7 @@ constexpr int kESlots = 4;  // E values staged per thread per plane (e_span <= 1
 
 // Base-block entry (r, block m, c) = K0[r][3m + c]: wave-uniform, compile-time
 // offsets, so it is fetched with scalar loads into SGPRs.
-#define ET_TILED_K(r, m, c) Kp[(r) * 24 + 3 * (m) + (c)]
+#define ET_TILED_K(r, m, c) K0[(r) * 24 + 3 * (m) + (c)]
 
 enum TiledMode { TM_Y = 0, TM_RES = 1, TM_JAC = 2, TM_JAC_ADD = 3 };
 
@@ -70,6 +70,16 @@ template <typename T>
 __host__ inline size_t tiled_lds_bytes(const TiledArgs& a) {
   return (size_t)4 * 3 * a.u_span * sizeof(T) + (size_t)3 * a.e_span * sizeof(T);
 }
+
+// Arguments of the element-centric fine-grid operators (matrix-core and
+// transformed-basis kernels): persistent grid over (x tile, node plane) units.
+struct MxArgs {
+  Grid g;
+  int64_t e_sx, e_sy;  // element index = ex*e_sx + ey*e_sy + ez
+  int n_xt;            // x tiles
+  int nzg;             // z groups of 16 elements (matrix-core kernel)
+  int64_t units;       // n_xt * ny
+};
 
 // offset bit of H8 corner a along axis d (0 = x, 1 = y, 2 = z)
 __host__ __device__ constexpr int h8_bit(int a, int d) { return d == 0 ? hx(a) : (d == 1 ? hy(a) : hz(a)); }
@@ -187,7 +197,7 @@ __global__ __launch_bounds__(256) void op0_tiled_kernel(const TiledArgs A, const
   store_e(j0);
   __syncthreads();
 
-  const T w = MODE >= TM_JAC ? (T)(*omega * wscale) : T(0);
+  const T w = MODE >= TM_JAC ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
   for (int j = j0; j < j1; ++j) {
     const bool more = j + 1 < j1;
     if (more) {
@@ -207,10 +217,7 @@ __global__ __launch_bounds__(256) void op0_tiled_kernel(const TiledArgs A, const
         if (MODE >= TM_JAC) dv[r] = Dinv[jb + pd + r];
       }
     }
-    // Re-read K0 from memory in every plane: keeps the 72 scalar-loaded values
-    // from being hoisted out of the plane loop (they would not fit in SGPRs).
-    const T* Kp = K0;
-    asm volatile("" : "+s"(Kp));
+
     if (active) {
       T s[8][3];
 #pragma unroll

@@ -1,0 +1,494 @@
+// Fine-grid H8 operator in the corner Walsh-Hadamard basis (FP32 or FP64
+// vector ALU), element-centric, marching in y.
+//
+// Algebra.  Number the corners of a brick element by their offset bits,
+// beta = hx + 2*hy + 4*hz, and let H be the 8-point Walsh-Hadamard matrix,
+// H[M][beta] = (-1)^popcount(M & beta).  The reflections of the brick (a
+// corner map beta -> beta ^ g together with a sign flip of the reflected
+// displacement components) leave K0 invariant, so in the transformed
+// coordinates  uh_c = H u_c  (per displacement component c) K0 is block
+// diagonal:
+//     yh_c[M] = sum_d G[M][c][d] * uh_d[M ^ e_c ^ e_d],   y_c = H yh_c,
+// with e_c = 1 << c and G[M][c][d] = (1/64) sum H[M][b] K0[(b,c),(b',d)]
+// H[M^e_c^e_d][b'].  G has 45 non-zero entries (block M = 0, the rigid
+// translations, vanishes; block 7 is diagonal; every other block misses the
+// pair of axes given by its bits or their complement), so an element product
+// costs 45 FMAs plus two separable transforms instead of 576 FMAs.
+//
+// The transforms are split into the x-z face stage (4-point, per node face)
+// and the y stage (sum / difference of the bottom and top faces).  Marching
+// in y, the top face of layer ey is the bottom face of layer ey+1: its
+// forward face transform and its transformed output are carried in
+// registers, so each element transforms one new face and finishes one face
+// per layer.
+//
+// Assembly without atomics, in a fixed order.  Thread = element; a
+// wavefront holds 64 elements consecutive along z.  The hz = 1 corner
+// contributions move one lane up (DPP wave_shr:1; lane 63's value goes to a
+// seam slot in LDS) and are added to the hz = 0 ones, giving per element the
+// two node contributions (hx = 0, 1) of node k = ez.  Those are written to
+// LDS and the node phase adds the contributions of element columns i and
+// i-1 (and the seam slot at k = 64, 128).
+//
+// Tiling.  A workgroup (8 wavefronts) owns TC element columns in x (16 for
+// nelz <= 64, 8 for nelz <= 128; all z), completing TC-1 node columns, and
+// marches in y; the (x tile, node plane) units are split evenly over a
+// persistent grid, each range starting with one extra layer that seeds the
+// carries.  LDS holds two staged node planes (SoA, constrained DOFs zeroed),
+// one layer of E and the per-column node contributions; the next plane and
+// layer are loaded into registers while the current layer is computed.
+#include "common.hpp"
+#include "op_tiled.hpp"
+#include "stiffness.hpp"
+
+namespace et {
+
+constexpr int kWThreads = 512;
+constexpr int kWSlots = 2;   // elements per thread per layer (16 slices of 64)
+constexpr int kWStage = 3;   // staged nodes per thread per plane ((TC+1)*nz <= 1536)
+constexpr int kWEStage = 2;  // staged E values per thread per layer (TC*nelz <= 1024)
+constexpr int kWNodes = 2;   // output nodes per thread per plane ((TC-1)*nz <= 1024)
+constexpr int kWG = 63;      // transformed blocks M = 1..7, 3 x 3 each
+
+// structural non-zeros of G[M][c][d] (M = 1..7)
+__host__ __device__ constexpr bool wht_nz(int M, int c, int d) {
+  if (M == 0) return false;
+  if (c == d) return true;
+  if (M == 7) return false;
+  const int pair = (1 << c) | (1 << d);
+  const int bits = __builtin_popcount(M);
+  // single-bit M: zero on the pair of the other two axes; two-bit M: zero on its own pair
+  return bits == 1 ? (pair & M) != 0 : pair != M;
+}
+
+__host__ inline int wht_tc(const Grid& g) { return g.ez <= 64 ? 16 : 8; }
+
+__host__ inline size_t wht_lds_values(const Grid& g) {
+  const int tc = wht_tc(g);
+  const size_t nz = g.nz(), span = (size_t)(tc + 1) * nz;
+  return 6 * span                 // two staged node planes
+         + 6 * (size_t)tc * nz    // node contributions [3][2][tc][nz]
+         + 6 * (size_t)tc * 2     // seam slots [3][2][tc][2]
+         + (size_t)tc * g.ez      // E of one layer
+         + 64;                    // G
+}
+
+// lane j takes lane j-1 of the wavefront, lane 0 takes 0
+__device__ inline unsigned wht_shr32(unsigned v) { return __builtin_amdgcn_update_dpp(0u, v, 0x138, 0xf, 0xf, false); }
+__device__ inline float wht_shr(float v) {
+  return __builtin_bit_cast(float, wht_shr32(__builtin_bit_cast(unsigned, v)));
+}
+__device__ inline double wht_shr(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = wht_shr32((unsigned)u), hi = wht_shr32((unsigned)(u >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+__device__ inline float wht_uniform(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ inline double wht_uniform(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// 4-point transform over (hx, hz): F[px + 2 pz] = sum (-1)^(px hx + pz hz) u[hx][hz]
+template <typename T>
+__device__ inline void wht_face(const T u00, const T u10, const T u01, const T u11, T* F) {
+  const T a0 = u00 + u10, a1 = u00 - u10, b0 = u01 + u11, b1 = u01 - u11;
+  F[0] = a0 + b0;
+  F[1] = a1 + b1;
+  F[2] = a0 - b0;
+  F[3] = a1 - b1;
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kWThreads) void op0_wht_kernel(const MxArgs A, const T* __restrict__ K0,
+                                                            const T* __restrict__ E,
+                                                            const uint8_t* __restrict__ cons,
+                                                            const T* __restrict__ x,
+                                                            const T* __restrict__ b,
+                                                            const T* __restrict__ Dinv,
+                                                            const double* __restrict__ omega,
+                                                            double wscale, const T* __restrict__ xadd,
+                                                            T* __restrict__ y) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const Grid& g = A.g;
+  const int nx = g.nx(), ny = g.ny(), nz = g.nz();
+  const int nelx = g.ex, nely = g.ey, nelz = g.ez;
+  const int spc = nelz <= 64 ? 1 : 2;  // 64-element slices per element column
+  const int tc = 16 / spc;             // element columns per x tile
+  const int span = (tc + 1) * nz;
+  T* su = reinterpret_cast<T*>(lds_raw);  // [2 slots][3 comps][span]
+  T* sw = su + 6 * span;                  // [3 comps][2 hx][tc][nz]
+  T* ss = sw + 6 * tc * nz;               // seams [3 comps][2 hx][tc][2]
+  T* se = ss + 12 * tc;                   // [tc][nelz]
+  T* sg = se + tc * nelz;                 // G, 63 entries
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t plane = (int64_t)nx * nz;
+
+  // G[M][c][d] from K0 (corner order of K0 is the H8 local order)
+  if (tid < kWG) {
+    const int M = 1 + tid / 9, c = (tid / 3) % 3, d = tid % 3;
+    const int Mp = M ^ (1 << c) ^ (1 << d);
+    double s = 0.0;
+    for (int bb = 0; bb < 8; ++bb) {
+      const int a = h8_corner(bb & 1, (bb >> 1) & 1, bb >> 2);
+      for (int bp = 0; bp < 8; ++bp) {
+        const int ap = h8_corner(bp & 1, (bp >> 1) & 1, bp >> 2);
+        const int sg1 = __builtin_popcount(M & bb) & 1, sg2 = __builtin_popcount(Mp & bp) & 1;
+        const double k = (double)K0[(3 * a + c) * 24 + 3 * ap + d];
+        s += (sg1 ^ sg2) ? -k : k;
+      }
+    }
+    sg[tid] = (T)(s / 64.0);
+  }
+  __syncthreads();
+  // the 45 non-zero coefficients are wave-uniform: scalar registers
+  T gk[kWG];
+#pragma unroll
+  for (int t = 0; t < kWG; ++t) {
+    const int M = 1 + t / 9, c = (t / 3) % 3, d = t % 3;
+    gk[t] = T(0);
+    if (wht_nz(M, c, d)) gk[t] = wht_uniform(sg[t]);
+  }
+  auto gval = [&](int t) -> T { return gk[t]; };
+  const T w = MODE >= TM_JAC ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
+
+  // element slots: slice s = wave + 8 m -> column s / spc, z range 64 (s % spc) + [0, 64)
+  int scl[kWSlots], sez[kWSlots];
+#pragma unroll
+  for (int m = 0; m < kWSlots; ++m) {
+    const int s = wave + 8 * m;
+    scl[m] = s / spc;
+    sez[m] = 64 * (s % spc) + lane;
+  }
+  // staged-node and E slots of this thread
+  int snl[kWStage], sk[kWStage], ecl[kWEStage], eez[kWEStage];
+#pragma unroll
+  for (int m = 0; m < kWStage; ++m) {
+    const int s = tid + kWThreads * m;
+    snl[m] = s / nz;
+    sk[m] = s - snl[m] * nz;
+  }
+#pragma unroll
+  for (int m = 0; m < kWEStage; ++m) {
+    const int t = tid + kWThreads * m;
+    ecl[m] = t / nelz;
+    eez[m] = t - ecl[m] * nelz;
+  }
+
+  // output nodes of this thread: staged slot s = nz + tid + 512 q (output columns start at 1)
+  int osl[kWNodes], onl[kWNodes], ok_[kWNodes];
+#pragma unroll
+  for (int q = 0; q < kWNodes; ++q) {
+    osl[q] = nz + tid + kWThreads * q;
+    onl[q] = osl[q] / nz;
+    ok_[q] = osl[q] - onl[q] * nz;
+  }
+
+  const int64_t u0 = (int64_t)blockIdx.x * A.units / gridDim.x;
+  const int64_t u1 = (int64_t)(blockIdx.x + 1) * A.units / gridDim.x;
+  int64_t un = u0;
+  while (un < u1) {
+    const int xt = (int)(un / ny);
+    const int ja = (int)(un - (int64_t)xt * ny);
+    const int jb = (int)min((int64_t)ny, (int64_t)ja + (u1 - un));
+    un += jb - ja;
+    const int c0 = xt * (tc - 1) - 1;  // element column of local column 0 = node column of staged column 0
+
+    T pu[kWStage][3];
+    T pe[kWEStage];
+    auto fetch_u = [&](int jp) {
+      const bool pin = jp >= 0 && jp < ny;
+      const int64_t pb = (int64_t)(pin ? jp : 0) * plane * 3 + (int64_t)c0 * nz * 3;
+#pragma unroll
+      for (int m = 0; m < kWStage; ++m) {
+        const int s = tid + kWThreads * m;
+        const int n = c0 + snl[m];
+        const bool in = pin && s < span && n >= 0 && n < nx;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          T v = T(0);
+          if (in) {
+            const int64_t d = pb + (int64_t)s * 3 + r;
+            v = x[d];
+            if (cons != nullptr && cons[d]) v = T(0);
+          }
+          pu[m][r] = v;
+        }
+      }
+    };
+    auto store_u = [&](int jp) {
+      T* dst = su + (size_t)(jp & 1) * 3 * span;
+#pragma unroll
+      for (int m = 0; m < kWStage; ++m) {
+        const int s = tid + kWThreads * m;
+        if (s < span) {
+#pragma unroll
+          for (int r = 0; r < 3; ++r) dst[r * span + s] = pu[m][r];
+        }
+      }
+    };
+    auto fetch_e = [&](int ey) {
+      const bool lin = ey >= 0 && ey < nely;
+#pragma unroll
+      for (int m = 0; m < kWEStage; ++m) {
+        const int c = c0 + ecl[m];
+        const bool in = lin && ecl[m] < tc && c >= 0 && c < nelx;
+        pe[m] = in ? E[(int64_t)c * A.e_sx + (int64_t)ey * A.e_sy + eez[m]] : T(0);
+      }
+    };
+    auto store_e = [&]() {
+#pragma unroll
+      for (int m = 0; m < kWEStage; ++m) {
+        const int t = tid + kWThreads * m;
+        if (t < tc * nelz) se[t] = pe[m];
+      }
+    };
+    // face transform of the face of plane jp (slot jp & 1) for element slot m
+    auto face = [&](int jp, int m, T (&F)[3][4]) {
+      const int ezc = min(sez[m], nelz - 1);
+      const T* ub = su + (size_t)(jp & 1) * 3 * span + scl[m] * nz + ezc;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T* uc = ub + c * span;
+        wht_face(uc[0], uc[nz], uc[1], uc[nz + 1], F[c]);
+      }
+    };
+
+    // prologue: node planes ja-1 and ja, element layer ja-1
+    fetch_u(ja - 1);
+    store_u(ja - 1);
+    fetch_u(ja);
+    store_u(ja);
+    fetch_e(ja - 1);
+    store_e();
+    __syncthreads();
+
+    // FP32 carries the transformed bottom face from the previous layer; FP64
+    // (twice the registers) recomputes it from the staged plane.
+    constexpr bool kCarryF = sizeof(T) == 4;
+    T Fb[kCarryF ? kWSlots : 1][3][4];  // transformed bottom face of the current layer
+    T Yc[kWSlots][3][4];                // transformed top-face output of the previous layer
+#pragma unroll
+    for (int m = 0; m < kWSlots; ++m) {
+      if (kCarryF) face(ja - 1, m, Fb[kCarryF ? m : 0]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int f = 0; f < 4; ++f) Yc[m][c][f] = T(0);
+    }
+
+    for (int ey = ja - 1; ey < jb; ++ey) {
+      const bool out = ey >= ja;
+      const bool more = ey + 2 <= jb;
+      if (more) {
+        fetch_u(ey + 2);
+        fetch_e(ey + 1);
+      }
+      // global operands of this layer's output plane, requested before the element phase
+      const int64_t jb3 = (int64_t)ey * plane * 3 + (int64_t)c0 * nz * 3;
+      bool act[kWNodes];
+      int64_t dd[kWNodes];
+      T bq[kWNodes][3], dq[kWNodes][3], aq[kWNodes][3], xq[kWNodes][3];
+      uint8_t cq[kWNodes][3];
+      // the node's own (staged, constrained-zeroed) value is read here, before the
+      // barrier: after it, the slot of plane ey is refilled by other threads
+      const T* own = su + (size_t)(ey & 1) * 3 * span;
+#pragma unroll
+      for (int q = 0; q < kWNodes; ++q) {
+        act[q] = out && onl[q] < tc && c0 + onl[q] < nx;
+        dd[q] = act[q] ? jb3 + (int64_t)osl[q] * 3 : 0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          xq[q][r] = (MODE != TM_Y && act[q]) ? own[r * span + osl[q]] : T(0);
+          cq[q][r] = cons != nullptr ? cons[dd[q] + r] : (uint8_t)0;
+          bq[q][r] = MODE != TM_Y ? b[dd[q] + r] : T(0);
+          dq[q][r] = MODE >= TM_JAC ? Dinv[dd[q] + r] : T(0);
+          aq[q][r] = MODE == TM_JAC_ADD ? xadd[dd[q] + r] : T(0);
+        }
+      }
+
+      // element phase: layer ey
+#pragma unroll
+      for (int m = 0; m < kWSlots; ++m) {
+        const int cl = scl[m], ez = sez[m];
+        T Ft[3][4];
+        face(ey + 1, m, Ft);
+        T (&Fbm)[3][4] = Fb[kCarryF ? m : 0];
+        if (!kCarryF) face(ey, m, Fbm);
+        const T e = ez < nelz ? se[cl * nelz + ez] : T(0);
+        T uh[3][8];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            const int M0 = (f & 1) | ((f >> 1) << 2);  // px + 4 pz
+            uh[c][M0] = e * (Fbm[c][f] + Ft[c][f]);
+            uh[c][M0 | 2] = e * (Fbm[c][f] - Ft[c][f]);
+            Fbm[c][f] = Ft[c][f];
+          }
+        T yh[3][8];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          yh[c][0] = T(0);
+#pragma unroll
+          for (int M = 1; M < 8; ++M) {
+            T s = T(0);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+              if (wht_nz(M, c, d)) s = fma(gval((M - 1) * 9 + c * 3 + d), uh[d][M ^ (1 << c) ^ (1 << d)], s);
+            yh[c][M] = s;
+          }
+        }
+        // y stage of the inverse; this layer's bottom face completes node plane ey
+        T v[3][2][2];  // [c][hx][hz]
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          T Yb[4];
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            const int M0 = (f & 1) | ((f >> 1) << 2);
+            Yb[f] = yh[c][M0] + yh[c][M0 | 2] + Yc[m][c][f];
+            Yc[m][c][f] = yh[c][M0] - yh[c][M0 | 2];
+          }
+          T o[4];
+          wht_face(Yb[0], Yb[1], Yb[2], Yb[3], o);  // symmetric; the normalisation is in G
+          v[c][0][0] = o[0];
+          v[c][1][0] = o[1];
+          v[c][0][1] = o[2];
+          v[c][1][1] = o[3];
+        }
+        // z: the hz = 1 contributions move one lane up; lane 63's go to the seam slot
+        const int k = ez;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int hx = 0; hx < 2; ++hx) {
+            const T up = v[c][hx][1];
+            const T zv = v[c][hx][0] + wht_shr(up);
+            if (out && k <= nelz) sw[((c * 2 + hx) * tc + cl) * nz + k] = zv;
+            if (out && lane == 63) ss[((c * 2 + hx) * tc + cl) * 2 + (ez >> 6)] = up;
+          }
+      }
+      __syncthreads();
+
+      // node phase: node plane ey
+      if (out) {
+#pragma unroll
+        for (int q = 0; q < kWNodes; ++q) {
+          if (act[q]) {
+            const int nl = onl[q], k = ok_[q];
+            const bool wv = k < 64 * spc;  // contribution slot written by a lane
+            const bool seam = k > 0 && (k & 63) == 0;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              const int64_t d = dd[q] + r;
+              T ax0 = T(0);
+              if (wv) ax0 = sw[((r * 2) * tc + nl) * nz + k] + sw[((r * 2 + 1) * tc + nl - 1) * nz + k];
+              if (seam)
+                ax0 += ss[((r * 2) * tc + nl) * 2 + (k >> 6) - 1] + ss[((r * 2 + 1) * tc + nl - 1) * 2 + (k >> 6) - 1];
+              const bool cr = cq[q][r] != 0;
+              T xv = xq[q][r];  // zero on constrained DOFs
+              if (cr) xv = x[d];
+              const T ax = cr ? xv : ax0;
+              T vv;
+              if (MODE == TM_Y)
+                vv = ax;
+              else if (MODE == TM_RES)
+                vv = bq[q][r] - ax;
+              else
+                vv = xv + w * dq[q][r] * (bq[q][r] - ax);
+              if (MODE == TM_JAC_ADD) vv += aq[q][r];
+              y[d] = vv;
+            }
+          }
+        }
+      }
+      if (more) {
+        store_u(ey + 2);
+        store_e();
+      }
+      __syncthreads();
+    }
+  }
+}
+
+static int g_wht_enable = 1;
+static int g_wht_wgs = 1;
+
+void set_wht(int on) { g_wht_enable = on ? 1 : 0; }
+int get_wht() { return g_wht_enable; }
+void set_wht_wgs(int n) { g_wht_wgs = n > 0 ? n : 1; }
+int get_wht_wgs() { return g_wht_wgs; }
+
+constexpr size_t kWMaxLds = 160 * 1024;
+
+bool wht_supported(const Grid& g, size_t elem_bytes) {
+  if (g.dim != 3 || g.ez > 128) return false;
+  const int tc = wht_tc(g);
+  return (tc + 1) * g.nz() <= kWThreads * kWStage && tc * g.ez <= kWThreads * kWEStage &&
+         (tc - 1) * g.nz() <= kWThreads * kWNodes && wht_lds_values(g) * elem_bytes <= kWMaxLds;
+}
+
+static int wht_device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
+
+template <typename T, int MODE>
+static void launch_wht(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const T* K0, const T* E,
+                       const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
+                       double wscale, const T* xadd, T* y) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&op0_wht_kernel<T, MODE>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWMaxLds);
+    attr_set = true;
+  }
+  op0_wht_kernel<T, MODE><<<n_wg, kWThreads, lds, s>>>(A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y);
+}
+
+template <typename T>
+int op0_wht(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E,
+            const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
+            double wscale, const T* xadd, T* y, hipStream_t s) {
+  ET_REQUIRE(wht_supported(g, sizeof(T)), "transformed-basis operator: 3-D grids with nelz <= 128 only");
+  MxArgs A;
+  A.g = g;
+  A.e_sx = e_sx;
+  A.e_sy = e_sy;
+  const int tc = wht_tc(g);
+  A.n_xt = (g.nx() + tc - 2) / (tc - 1);
+  A.nzg = 0;
+  A.units = (int64_t)A.n_xt * g.ny();
+  const size_t lds = wht_lds_values(g) * sizeof(T);
+  const int64_t slots = (int64_t)wht_device_cus() * g_wht_wgs;
+  const int n_wg = (int)(A.units < slots ? A.units : slots);
+  switch (mode) {
+    case TM_Y: launch_wht<T, TM_Y>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
+    case TM_RES: launch_wht<T, TM_RES>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
+    case TM_JAC: launch_wht<T, TM_JAC>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
+    default: launch_wht<T, TM_JAC_ADD>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
+  }
+  ET_CHECK_LAUNCH();
+  return ET_OK;
+}
+
+template int op0_wht<double>(int, const Grid&, int64_t, int64_t, const double*, const double*,
+                             const uint8_t*, const double*, const double*, const double*,
+                             const double*, double, const double*, double*, hipStream_t);
+template int op0_wht<float>(int, const Grid&, int64_t, int64_t, const float*, const float*,
+                            const uint8_t*, const float*, const float*, const float*,
+                            const double*, double, const float*, float*, hipStream_t);
+
+}  // namespace et

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void op0_kernel(const Grid g, const T* __restr
       const int64_t m = DIM == 3 ? ((int64_t)by * nx + bx) * nz + bz : (int64_t)by * nx + bx;
       T um[DIM];
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) um[c] = cons[m * DIM + c] ? T(0) : x[m * DIM + c];
+      for (int c = 0; c < DIM; ++c) um[c] = (cons != nullptr && cons[m * DIM + c]) ? T(0) : x[m * DIM + c];
 #pragma unroll
       for (int r = 0; r < DIM; ++r)
 #pragma unroll
@@ -99,12 +99,16 @@ __global__ __launch_bounds__(256) void op0_kernel(const Grid g, const T* __restr
   }
 #pragma unroll
   for (int r = 0; r < DIM; ++r)
-    if (cons[n * DIM + r]) acc[r] = x[n * DIM + r];
-  const T w = MODE >= OP_JAC ? (T)(*omega * wscale) : T(0);
+    if (cons != nullptr && cons[n * DIM + r]) acc[r] = x[n * DIM + r];
+  const T w = MODE >= OP_JAC ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
   op_epilogue<T, DIM, MODE>(n, acc, x, b, Dinv, w, xadd, y);
 }
 
-// Coarse level operator: block stencil.
+// Coarse level operator: symmetric block stencil stored by halves.  S holds
+// the centre and the forward offsets o = centre+1 .. 26 (3-D; 5 .. 8 in 2-D)
+// of the full offset numbering o = (di+1)*9 + (dj+1)*3 + (dk+1); the backward
+// blocks follow from symmetry, A(n, n-o) = A(n-o, n)^T = S_o(n-o)^T.
+// Layout S[((h*dof + r)*dof + c)*N + n], h = o - centre.
 template <typename T, int DIM, int MODE>
 __global__ __launch_bounds__(256) void opS_kernel(const Grid g, const T* __restrict__ S,
                                                   const T* __restrict__ x, const T* __restrict__ b,
@@ -112,11 +116,12 @@ __global__ __launch_bounds__(256) void opS_kernel(const Grid g, const T* __restr
                                                   const double* __restrict__ omega, double wscale,
                                                   const T* __restrict__ xadd,
                                                   T* __restrict__ y) {
-  constexpr int NO = DIM == 3 ? 27 : 9;
+  constexpr int OC = DIM == 3 ? 13 : 4;  // centre offset
+  constexpr int NH = OC + 1;             // stored blocks per node
   const int64_t N = g.n_nodes();
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
-  const int nz = g.nz(), nx = g.nx();
+  const int nz = g.nz(), nx = g.nx(), ny = g.ny();
   const int k = DIM == 3 ? (int)(n % nz) : 0;
   const int64_t t = DIM == 3 ? n / nz : n;
   const int i = (int)(t % nx), j = (int)(t / nx);
@@ -124,22 +129,42 @@ __global__ __launch_bounds__(256) void opS_kernel(const Grid g, const T* __restr
 #pragma unroll
   for (int r = 0; r < DIM; ++r) acc[r] = T(0);
 #pragma unroll
-  for (int o = 0; o < NO; ++o) {
+  for (int h = 0; h < NH; ++h) {
+    const int o = OC + h;
     const int oi = DIM == 3 ? o / 9 - 1 : o / 3 - 1;
     const int oj = DIM == 3 ? (o / 3) % 3 - 1 : o % 3 - 1;
     const int ok = DIM == 3 ? o % 3 - 1 : 0;
-    const int mi = i + oi, mj = j + oj, mk = k + ok;
-    if (mi < 0 || mi >= nx || mj < 0 || mj >= g.ny()) continue;
-    if (DIM == 3 && (mk < 0 || mk >= nz)) continue;
-    const int64_t m = DIM == 3 ? ((int64_t)mj * nx + mi) * nz + mk : (int64_t)mj * nx + mi;
+    const int64_t off = DIM == 3 ? ((int64_t)oj * nx + oi) * nz + ok : (int64_t)oj * nx + oi;
+    // forward block S_o(n) x_{n+o}
+    {
+      const int mi = i + oi, mj = j + oj, mk = k + ok;
+      const bool in = mi >= 0 && mi < nx && mj >= 0 && mj < ny && (DIM == 2 || (mk >= 0 && mk < nz));
+      if (in) {
+        const int64_t m = n + off;
 #pragma unroll
-    for (int q = 0; q < DIM; ++q) {
-      const T xv = x[m * DIM + q];
+        for (int q = 0; q < DIM; ++q) {
+          const T xv = x[m * DIM + q];
 #pragma unroll
-      for (int r = 0; r < DIM; ++r) acc[r] += S[((int64_t)(o * DIM + r) * DIM + q) * N + n] * xv;
+          for (int r = 0; r < DIM; ++r) acc[r] += S[((int64_t)(h * DIM + r) * DIM + q) * N + n] * xv;
+        }
+      }
+    }
+    // backward block S_o(n-o)^T x_{n-o}
+    if (h > 0) {
+      const int mi = i - oi, mj = j - oj, mk = k - ok;
+      const bool in = mi >= 0 && mi < nx && mj >= 0 && mj < ny && (DIM == 2 || (mk >= 0 && mk < nz));
+      if (in) {
+        const int64_t m = n - off;
+#pragma unroll
+        for (int q = 0; q < DIM; ++q) {
+          const T xv = x[m * DIM + q];
+#pragma unroll
+          for (int r = 0; r < DIM; ++r) acc[r] += S[((int64_t)(h * DIM + q) * DIM + r) * N + m] * xv;
+        }
+      }
     }
   }
-  const T w = MODE >= OP_JAC ? (T)(*omega * wscale) : T(0);
+  const T w = MODE >= OP_JAC ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
   op_epilogue<T, DIM, MODE>(n, acc, x, b, Dinv, w, xadd, y);
 }
 
@@ -181,13 +206,21 @@ __global__ void stencil_dinv_kernel(int64_t N, const double* __restrict__ S, T* 
   for (int r = 0; r < DIM; ++r) out[n * DIM + r] = (T)(1.0 / S[((int64_t)(oc * DIM + r) * DIM + r) * N + n]);
 }
 
-// PCG: x += alpha p; r -= alpha q; partial sums of r.r   (alpha = rho / pq)
+// PCG: x += alpha p; r -= alpha q (alpha = rho / pq); optionally a working-
+// precision copy of r for the V-cycle; per-block partial sums of r.r.  A no-op
+// once the loop has stopped (SC_STOP).
+template <typename TB>
 __global__ __launch_bounds__(256) void pcg_update_xr_kernel(int64_t n, const double* __restrict__ sc,
                                                             const double* __restrict__ p,
                                                             const double* __restrict__ q,
                                                             double* __restrict__ x,
                                                             double* __restrict__ r,
+                                                            TB* __restrict__ rb,
                                                             double* __restrict__ partial) {
+  if (sc[SC_STOP] != 0.0) {
+    if (threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+    return;
+  }
   const double alpha = sc[SC_RHO] / sc[SC_PQ];
   double s = 0.0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
@@ -195,29 +228,80 @@ __global__ __launch_bounds__(256) void pcg_update_xr_kernel(int64_t n, const dou
     x[t] += alpha * p[t];
     const double rv = r[t] - alpha * q[t];
     r[t] = rv;
+    if (rb != nullptr) rb[t] = (TB)rv;
     s += rv * rv;
   }
   s = block_sum256(s);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// p = z + beta p with beta = rho_new / rho (Fletcher-Reeves); then rho <- rho_new
-__global__ __launch_bounds__(256) void pcg_update_p_kernel(int64_t n, const double* __restrict__ sc,
-                                                           const double* __restrict__ z,
-                                                           double* __restrict__ p) {
-  const double beta = sc[SC_RHO_NEW] / sc[SC_RHO];
+// per-block partial sums of r.z with z in the V-cycle's precision
+template <typename TZ>
+__global__ __launch_bounds__(256) void pcg_dot_kernel(int64_t n, const double* __restrict__ r,
+                                                      const TZ* __restrict__ z,
+                                                      double* __restrict__ partial) {
+  double s = 0.0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x)
-    p[t] = z[t] + beta * p[t];
+    s += r[t] * (double)z[t];
+  s = block_sum256(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// p = z + beta p with beta = rho_new / rho (Fletcher-Reeves); p = z when first
+template <typename TZ>
+__global__ __launch_bounds__(256) void pcg_update_p_kernel(int64_t n, const double* __restrict__ sc,
+                                                           const TZ* __restrict__ z,
+                                                           double* __restrict__ p, int first) {
+  const double beta = first ? 0.0 : sc[SC_RHO_NEW] / sc[SC_RHO];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = first ? (double)z[t] : (double)z[t] + beta * p[t];
 }
 
 __global__ void pcg_rotate_kernel(double* sc) { sc[SC_RHO] = sc[SC_RHO_NEW]; }
 
+// Convergence / breakdown test of one iteration, evaluated on the device so the
+// host never waits on the current iteration: R = |r| / |b|; non-finite p.q or
+// r.r, p.q <= 0 and R < tol stop the loop (first reason wins, as in a plain
+// sequential loop).  {R, status, stop} is published to pinned host memory.
+__global__ void pcg_check_kernel(double* sc, double tol, double* ring) {
+  if (sc[SC_STOP] == 0.0) {
+    const double pq = sc[SC_PQ], rr = sc[SC_RR];
+    const double R = sqrt(rr) / sqrt(sc[SC_BB]);
+    double status = (double)ET_OK, stop = 0.0;
+    if (!isfinite(pq) || !isfinite(rr)) {
+      status = (double)ET_ENONFINITE;
+      stop = 1.0;
+    } else if (!(pq > 0.0)) {
+      status = (double)ET_EBREAKDOWN;
+      stop = 1.0;
+    } else if (R < tol) {
+      stop = 1.0;
+    }
+    sc[SC_R] = R;
+    sc[SC_STATUS] = status;
+    sc[SC_STOP] = stop;
+  }
+  ring[0] = sc[SC_R];
+  ring[1] = sc[SC_STATUS];
+  ring[2] = sc[SC_STOP];
+  __threadfence_system();
+}
+
 // omega = min + d/2 exp((-clip(R, tol, 0.1) + tol) * 500), R = |r| / |b|
 __global__ void pcg_omega_kernel(double* sc, double tol, double wmin, double dw) {
+  if (sc[SC_STOP] != 0.0) return;
   const double R = sqrt(sc[SC_RR]) / sqrt(sc[SC_BB]);
   const double Rc = fmin(fmax(R, tol), 1e-1);
   sc[SC_OMEGA] = wmin + dw / 2.0 * exp((-Rc + tol) * 500.0);
+}
+
+__global__ void pcg_init_kernel(double* sc, double omega) {
+  sc[SC_OMEGA] = omega;
+  sc[SC_R] = 0.0;
+  sc[SC_STATUS] = 0.0;
+  sc[SC_STOP] = 0.0;
 }
 
 __global__ void set_scalar_kernel(double* sc, int slot, double v) { sc[slot] = v; }
@@ -242,7 +326,7 @@ static int op_launch(const MGLevel<T>& L, const MGShared& sh, int mode, const T*
     return st;
   }
   if (L.level == 0 && sh.tiled) {
-    const int st = op0_tiled<T>(mode, g, (int64_t)g.ey * g.ez, g.ez, sh.K0<T>(), sh.E<T>(), sh.cons,
+    const int st = op0_fine<T>(mode, g, (int64_t)g.ey * g.ez, g.ez, sh.K0<T>(), sh.E<T>(), sh.cons,
                                 x, b, L.Dinv, omega, wscale, xadd, y, s);
     sh.prof->end(s);
     return st;
@@ -346,8 +430,8 @@ int MGHierarchy<T>::alloc(const Grid& g0, int dof_, int n_level_, hipStream_t s)
     const size_t vb = (size_t)L.n * sizeof(T);
     for (T** p : {&L.b, &L.x, &L.r, &L.e, &L.t, &L.Dinv}) ET_CHECK_HIP(hipMalloc(p, vb));
     if (l > 0 && l < n_level) {
-      const int NO = g.dim == 3 ? 27 : 9;
-      ET_CHECK_HIP(hipMalloc(&L.S, (size_t)NO * dof * dof * g.n_nodes() * sizeof(T)));
+      const int NH = g.dim == 3 ? 14 : 5;  // centre + forward offsets
+      ET_CHECK_HIP(hipMalloc(&L.S, (size_t)NH * dof * dof * g.n_nodes() * sizeof(T)));
     }
     g = g.coarse();
   }
@@ -365,6 +449,43 @@ void MGHierarchy<T>::release() {
 
 template struct MGHierarchy<double>;
 template struct MGHierarchy<float>;
+
+// Level-0 operator outside the solver (C ABI): the fine-grid kernels in 3-D,
+// the node-gather kernel otherwise.
+template <typename T>
+int level0_op(int mode, const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T* x,
+              const T* b, const T* Dinv, const double* omega, double wscale, const T* xadd, T* y,
+              hipStream_t s) {
+  if (g.dim == 3 && fine_supported(g, sizeof(T)))
+    return op0_fine<T>(mode, g, (int64_t)g.ey * g.ez, g.ez, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, s);
+  const unsigned grid = grid_for(g.n_nodes());
+#define ET_L0(DIM, MODE) \
+  op0_kernel<T, DIM, MODE><<<grid, kBlock, 0, s>>>(g, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y)
+#define ET_L0D(MODE)     \
+  do {                   \
+    if (g.dim == 3)      \
+      ET_L0(3, MODE);    \
+    else                 \
+      ET_L0(2, MODE);    \
+  } while (0)
+  switch (mode) {
+    case OP_Y: ET_L0D(OP_Y); break;
+    case OP_RES: ET_L0D(OP_RES); break;
+    case OP_JAC: ET_L0D(OP_JAC); break;
+    default: ET_L0D(OP_JAC_ADD); break;
+  }
+#undef ET_L0D
+#undef ET_L0
+  ET_CHECK_LAUNCH();
+  return ET_OK;
+}
+
+template int level0_op<double>(int, const Grid&, const double*, const double*, const uint8_t*,
+                               const double*, const double*, const double*, const double*, double,
+                               const double*, double*, hipStream_t);
+template int level0_op<float>(int, const Grid&, const float*, const float*, const uint8_t*,
+                              const float*, const float*, const float*, const double*, double,
+                              const float*, float*, hipStream_t);
 
 // ------------------------------------------------------------ et_mg ----
 
@@ -404,7 +525,7 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
   }
   n_dof = g0.n_nodes() * dof;
   n_elem = g0.n_elem();
-  sh.tiled = g0.dim == 3 && tiled_supported(g0, sizeof(double));
+  sh.tiled = g0.dim == 3 && fine_supported(g0, sizeof(double)) && fine_supported(g0, sizeof(float));
   sh.mfma = p.precision == 1 && get_mixed_mfma() && mfma_supported(g0);
   sh.kscale = k0_scale(K0_host, (p.dim == 3 ? 24 * 24 : 8 * 8));
   const int nd = (p.dim == 3 ? 8 : 4) * dof;
@@ -419,7 +540,9 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
   }
   ET_CHECK_HIP(hipMalloc(&sh.scal, SC_COUNT * sizeof(double)));
   ET_CHECK_HIP(hipMalloc(&partial, kRedBlocks * sizeof(double)));
-  ET_CHECK_HIP(hipHostMalloc(&host_scal, SC_COUNT * sizeof(double)));
+  ET_CHECK_HIP(hipHostMalloc(&ring, 8 * sizeof(double), hipHostMallocCoherent));
+  ET_CHECK_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&ring_dev), ring, 0));
+  for (auto& e : ring_ev) ET_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (double** v : {&r, &z, &p_, &q, &diag64}) ET_CHECK_HIP(hipMalloc(v, n_dof * sizeof(double)));
   if (p.precision == 1) ET_CHECK_HIP(hipMalloc(&sh.Ef, n_elem * sizeof(float)));
   // FP64 stencils: level 0 (fine operator) and every coarse level
@@ -470,7 +593,9 @@ void MGSolver::destroy() {
   for (void* v : {(void*)sh.K0d, (void*)sh.K0f, (void*)sh.scal, (void*)partial, (void*)r, (void*)z,
                   (void*)p_, (void*)q, (void*)diag64, (void*)sh.Ef, (void*)Mt, (void*)eflag})
     if (v) (void)hipFree(v);
-  if (host_scal) (void)hipHostFree(host_scal);
+  if (ring) (void)hipHostFree(ring);
+  for (auto& e : ring_ev)
+    if (e) (void)hipEventDestroy(e);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (ev2) (void)hipEventDestroy(ev2);
@@ -503,9 +628,11 @@ int MGSolver::setup_levels(MGHierarchy<T>& h) {
       stencil_dinv_kernel<T, 2><<<grid_for(N), kBlock, 0, s>>>(N, S64[l], lev.Dinv);
     ET_CHECK_LAUNCH();
     if (l < L) {
-      const int NO = g.dim == 3 ? 27 : 9;
-      const int64_t cnt = (int64_t)NO * dof * dof * N;
-      convert_kernel<double, T><<<grid_for(cnt), kBlock, 0, s>>>(cnt, S64[l], lev.S);
+      // the centre and forward offsets are the trailing blocks of the full stencil
+      const int OC = g.dim == 3 ? 13 : 4;
+      const int64_t cnt = (int64_t)(OC + 1) * dof * dof * N;
+      convert_kernel<double, T><<<grid_for(cnt), kBlock, 0, s>>>(cnt, S64[l] + (int64_t)OC * dof * dof * N,
+                                                                 lev.S);
       ET_CHECK_LAUNCH();
     }
   }
@@ -514,19 +641,27 @@ int MGSolver::setup_levels(MGHierarchy<T>& h) {
 }
 
 template <typename T>
-int MGSolver::precondition(MGHierarchy<T>& h) {
-  // z = V(r): level-0 rhs is the PCG residual, result goes to z
+int MGSolver::precondition(MGHierarchy<T>& h, bool rhs_ready, const T** zout) {
+  // z = V(r).  FP64 V-cycles read r in place (level-0 b is only read);
+  // FP32 V-cycles take the FP32 copy of r that the x/r update wrote.
   hipStream_t s = stream;
   MGLevel<T>& L0 = h.lv[0];
-  prof.begin(ET_PROF_PCG_VEC, s);
-  convert_kernel<double, T><<<grid_for(n_dof), kBlock, 0, s>>>(n_dof, r, L0.b);
-  prof.end(s);
-  ET_CHECK_LAUNCH();
-  ET_TRY(h.vcycle(0, sh, s));
-  prof.begin(ET_PROF_PCG_VEC, s);
-  convert_kernel<T, double><<<grid_for(n_dof), kBlock, 0, s>>>(n_dof, L0.x, z);
-  prof.end(s);
-  ET_CHECK_LAUNCH();
+  if constexpr (sizeof(T) == sizeof(double)) {
+    T* saved = L0.b;
+    L0.b = r;
+    const int st = h.vcycle(0, sh, s);
+    L0.b = saved;
+    ET_TRY(st);
+  } else {
+    if (!rhs_ready) {
+      prof.begin(ET_PROF_PCG_VEC, s);
+      convert_kernel<double, T><<<grid_for(n_dof), kBlock, 0, s>>>(n_dof, r, L0.b);
+      prof.end(s);
+      ET_CHECK_LAUNCH();
+    }
+    ET_TRY(h.vcycle(0, sh, s));
+  }
+  *zout = L0.x;
   return ET_OK;
 }
 
@@ -535,7 +670,7 @@ int MGSolver::apply_A64(const double* x, double* y, int mode, const double* b) {
   const unsigned grid = grid_for(g0.n_nodes());
   prof.begin(ET_PROF_MATVEC64, s);
   if (sh.tiled) {
-    const int st = op0_tiled<double>(mode, g0, (int64_t)g0.ey * g0.ez, g0.ez, sh.K0d, sh.Ed, sh.cons,
+    const int st = op0_fine<double>(mode, g0, (int64_t)g0.ey * g0.ez, g0.ez, sh.K0d, sh.Ed, sh.cons,
                                      x, b, nullptr, sh.scal, 1.0, nullptr, y, s);
     prof.end(s);
     return st;
@@ -556,11 +691,6 @@ int MGSolver::apply_A64(const double* x, double* y, int mode, const double* b) {
   return ET_OK;
 }
 
-int MGSolver::read_scalars() {
-  ET_CHECK_HIP(hipMemcpyAsync(host_scal, sh.scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, stream));
-  ET_CHECK_HIP(hipStreamSynchronize(stream));
-  return ET_OK;
-}
 
 int MGSolver::solve(const double* E, const uint8_t* cons, const double* b, double* x, double tol,
                     int maxiter, int* iters, double* residual) {
@@ -583,54 +713,10 @@ int MGSolver::solve(const double* E, const uint8_t* cons, const double* b, doubl
   ET_TRY(coarse.check_info(s));
   ET_CHECK_HIP(hipEventRecord(ev1, s));
   // omega <- min_omega at the start of every solve (GPU semantics)
-  set_scalar_kernel<<<1, 1, 0, s>>>(sh.scal, SC_OMEGA, min_omega);
+  pcg_init_kernel<<<1, 1, 0, s>>>(sh.scal, min_omega);
   ET_CHECK_LAUNCH();
-  // r = b - A x ; |b|^2
-  ET_TRY(apply_A64(x, r, OP_RES, b));
-  ET_TRY(dot(n_dof, b, b, partial, sh.scal + SC_BB, s));
-  auto prec = [&]() -> int {
-    return prm.precision == 1 ? precondition(h32) : precondition(h64);
-  };
-  ET_TRY(prec());
-  ET_CHECK_HIP(hipMemcpyAsync(p_, z, n_dof * sizeof(double), hipMemcpyDeviceToDevice, s));
-  ET_TRY(dot(n_dof, r, z, partial, sh.scal + SC_RHO, s));
-  int it = 0;
-  double R = 0.0;
-  int status = ET_OK;
-  for (it = 1; it <= maxiter; ++it) {
-    ET_TRY(apply_A64(p_, q, OP_Y, nullptr));
-    prof.begin(ET_PROF_PCG_VEC, s);
-    ET_TRY(dot(n_dof, p_, q, partial, sh.scal + SC_PQ, s));
-    pcg_update_xr_kernel<<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, p_, q, x, r, partial);
-    ET_CHECK_LAUNCH();
-    ET_TRY(reduce_finalize(partial, sh.scal + SC_RR, s));
-    prof.end(s);
-    ET_TRY(read_scalars());
-    const double pq = host_scal[SC_PQ], rr = host_scal[SC_RR], bb = host_scal[SC_BB];
-    R = std::sqrt(rr) / std::sqrt(bb);
-    if (!std::isfinite(pq) || !std::isfinite(rr)) {
-      status = ET_ENONFINITE;
-      set_error("PCG: non-finite value (p.q or r.r)");
-      break;
-    }
-    if (!(pq > 0.0)) {
-      status = ET_EBREAKDOWN;
-      set_error("PCG breakdown: p^T A p <= 0");
-      break;
-    }
-    if (R < tol) break;
-    pcg_omega_kernel<<<1, 1, 0, s>>>(sh.scal, tol, min_omega, d_omega);
-    ET_CHECK_LAUNCH();
-    ET_TRY(prec());
-    prof.begin(ET_PROF_PCG_VEC, s);
-    ET_TRY(dot(n_dof, r, z, partial, sh.scal + SC_RHO_NEW, s));
-    pcg_update_p_kernel<<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, z, p_);
-    ET_CHECK_LAUNCH();
-    pcg_rotate_kernel<<<1, 1, 0, s>>>(sh.scal);
-    prof.end(s);
-    ET_CHECK_LAUNCH();
-  }
-  if (it > maxiter) it = maxiter;
+  int status = prm.precision == 1 ? pcg(h32, b, x, tol, maxiter, iters, residual)
+                                  : pcg(h64, b, x, tol, maxiter, iters, residual);
   ET_CHECK_HIP(hipEventRecord(ev2, s));
   ET_CHECK_HIP(hipEventSynchronize(ev2));
   float a = 0.f, c = 0.f;
@@ -638,7 +724,74 @@ int MGSolver::solve(const double* E, const uint8_t* cons, const double* b, doubl
   ET_CHECK_HIP(hipEventElapsedTime(&c, ev1, ev2));
   t_setup_ms = a;
   t_pcg_ms = c;
-  if (iters) *iters = it;
+  return status;
+}
+
+// Preconditioned CG (Fletcher-Reeves, as the reference) with the convergence
+// test evaluated on the device.  The host enqueues iteration k+1 before it
+// reads the outcome of iteration k, so the GPU never idles on a host round
+// trip; once the device has stopped the loop, the already-enqueued work of the
+// next iteration leaves x and r untouched.
+template <typename T>
+int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int maxiter, int* iters,
+                   double* residual) {
+  hipStream_t s = stream;
+  T* rb = nullptr;
+  if constexpr (sizeof(T) == sizeof(float)) rb = h.lv[0].b;
+  // r = b - A x ; |b|^2
+  ET_TRY(apply_A64(x, r, OP_RES, b));
+  ET_TRY(dot(n_dof, b, b, partial, sh.scal + SC_BB, s));
+  const T* z = nullptr;
+  ET_TRY(precondition(h, false, &z));
+  prof.begin(ET_PROF_PCG_VEC, s);
+  pcg_dot_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, r, z, partial);
+  ET_TRY(reduce_finalize(partial, sh.scal + SC_RHO, s));
+  pcg_update_p_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, z, p_, 1);
+  prof.end(s);
+  ET_CHECK_LAUNCH();
+  int last = 0;
+  auto read_ring = [&](int k, double* R, int* st) -> bool {
+    ET_CHECK_HIP(hipEventSynchronize(ring_ev[k & 1]));
+    const double* slot = ring + 4 * (k & 1);
+    *R = slot[0];
+    *st = (int)slot[1];
+    return slot[2] != 0.0;
+  };
+  double R = 0.0;
+  int status = ET_OK;
+  bool stopped = false;
+  for (int it = 1; it <= maxiter; ++it) {
+    ET_TRY(apply_A64(p_, q, OP_Y, nullptr));
+    prof.begin(ET_PROF_PCG_VEC, s);
+    ET_TRY(dot(n_dof, p_, q, partial, sh.scal + SC_PQ, s));
+    pcg_update_xr_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, p_, q, x, r, rb, partial);
+    ET_CHECK_LAUNCH();
+    ET_TRY(reduce_finalize(partial, sh.scal + SC_RR, s));
+    pcg_check_kernel<<<1, 1, 0, s>>>(sh.scal, tol, ring_dev + 4 * (it & 1));
+    ET_CHECK_LAUNCH();
+    ET_CHECK_HIP(hipEventRecord(ring_ev[it & 1], s));
+    prof.end(s);
+    last = it;
+    if (it > 1 && read_ring(it - 1, &R, &status)) {
+      last = it - 1;
+      stopped = true;
+      break;
+    }
+    pcg_omega_kernel<<<1, 1, 0, s>>>(sh.scal, tol, min_omega, d_omega);
+    ET_CHECK_LAUNCH();
+    ET_TRY(precondition(h, true, &z));
+    prof.begin(ET_PROF_PCG_VEC, s);
+    pcg_dot_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, r, z, partial);
+    ET_TRY(reduce_finalize(partial, sh.scal + SC_RHO_NEW, s));
+    pcg_update_p_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, z, p_, 0);
+    pcg_rotate_kernel<<<1, 1, 0, s>>>(sh.scal);
+    prof.end(s);
+    ET_CHECK_LAUNCH();
+  }
+  if (!stopped) (void)read_ring(last, &R, &status);
+  if (status == ET_ENONFINITE) set_error("PCG: non-finite value (p.q or r.r)");
+  if (status == ET_EBREAKDOWN) set_error("PCG breakdown: p^T A p <= 0");
+  if (iters) *iters = last;
   if (residual) *residual = R;
   return status;
 }

@@ -8,7 +8,7 @@
 namespace et {
 
 // device scalar slots of the PCG loop
-enum { SC_BB = 0, SC_RHO, SC_RHO_NEW, SC_PQ, SC_RR, SC_OMEGA, SC_COUNT };
+enum { SC_BB = 0, SC_RHO, SC_RHO_NEW, SC_PQ, SC_RR, SC_OMEGA, SC_R, SC_STATUS, SC_STOP, SC_COUNT };
 
 // Event-based per-kernel-class timing (enabled on demand).
 struct Prof {
@@ -88,7 +88,7 @@ struct MGLevel {
   int64_t n = 0;  // DOFs on this level
   T *b = nullptr, *x = nullptr, *r = nullptr, *e = nullptr, *t = nullptr;
   T* Dinv = nullptr;
-  T* S = nullptr;  // block stencil (coarse smoothing levels only)
+  T* S = nullptr;  // centre + forward-offset half of the symmetric block stencil (coarse smoothing levels)
 };
 
 template <typename T>
@@ -118,7 +118,9 @@ struct MGSolver {
 
   double *r = nullptr, *z = nullptr, *p_ = nullptr, *q = nullptr, *diag64 = nullptr;
   double* partial = nullptr;
-  double* host_scal = nullptr;
+  double* ring = nullptr;      // pinned host slots {R, status, stop} of the last two iterations
+  double* ring_dev = nullptr;  // device address of ring
+  hipEvent_t ring_ev[2] = {nullptr, nullptr};
   double max_omega = 0, min_omega = 0, d_omega = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   double t_setup_ms = 0, t_pcg_ms = 0;
@@ -134,9 +136,11 @@ struct MGSolver {
   template <typename T>
   int setup_levels(MGHierarchy<T>& h);
   template <typename T>
-  int precondition(MGHierarchy<T>& h);
+  int precondition(MGHierarchy<T>& h, bool rhs_ready, const T** z);
+  template <typename T>
+  int pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int maxiter, int* iters,
+          double* residual);
   int apply_A64(const double* x, double* y, int mode, const double* b);
-  int read_scalars();
 };
 
 }  // namespace et

@@ -118,8 +118,8 @@ template <typename T>
 int matvec(const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T* u, T* y,
            hipStream_t s) {
   const unsigned grid = grid_for(g.n_nodes());
-  if (g.dim == 3 && tiled_supported(g, sizeof(T)))
-    return op0_tiled<T>(0, g, (int64_t)g.ey * g.ez, g.ez, K0, E, cons, u, nullptr, nullptr, nullptr,
+  if (g.dim == 3 && fine_supported(g, sizeof(T)))
+    return op0_fine<T>(0, g, (int64_t)g.ey * g.ez, g.ez, K0, E, cons, u, nullptr, nullptr, nullptr,
                         1.0, nullptr, y, s);
   if (g.dim == 3)
     matvec_kernel<T, 3><<<grid, kBlock, 0, s>>>(g, K0, E, cons, u, y);

@@ -21,6 +21,39 @@ int op0_tiled(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, 
               const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
               double wscale, const T* xadd, T* y, hipStream_t s);
 bool tiled_supported(const Grid& g, size_t elem_bytes);
+// Matrix-core variant at working precision (FP32 or FP64 MFMA), same modes.
+template <typename T>
+int op0_mx(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E,
+           const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
+           double wscale, const T* xadd, T* y, hipStream_t s);
+bool mx_supported(const Grid& g, size_t elem_bytes);
+void set_mx(int is_double, int on);
+int get_mx(int is_double);
+void set_mx_wgs(int is_double, int n);
+int get_mx_wgs(int is_double);
+// Transformed-basis (corner Walsh-Hadamard) element operator, FP32 or FP64, same modes.
+template <typename T>
+int op0_wht(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E,
+            const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
+            double wscale, const T* xadd, T* y, hipStream_t s);
+bool wht_supported(const Grid& g, size_t elem_bytes);
+void set_wht(int on);
+int get_wht();
+void set_wht_wgs(int n);
+int get_wht_wgs();
+// Fine-grid dispatch: the transformed-basis operator where enabled and
+// supported, else the matrix-core one where enabled, else the LDS-tiled one.
+template <typename T>
+int op0_fine(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E,
+             const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
+             double wscale, const T* xadd, T* y, hipStream_t s);
+bool fine_supported(const Grid& g, size_t elem_bytes);
+// Any level-0 operator mode on any grid (fine-grid kernels in 3-D when supported).
+// omega == nullptr means omega = 1.
+template <typename T>
+int level0_op(int mode, const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T* x,
+              const T* b, const T* Dinv, const double* omega, double wscale, const T* xadd, T* y,
+              hipStream_t s);
 // Matrix-core FP32 variant (FP16 hi/lo MFMA); kscale: power of two >= max|K0|.
 int op0_mfma(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, float kscale, const float* K0,
              const float* E, const uint8_t* cons, const float* x, const float* b, const float* Dinv,

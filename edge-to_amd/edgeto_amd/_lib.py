@@ -46,6 +46,10 @@ _SIGS = {
     "et_matvec_f64": [_I, _IP, _P, _P, _P, _P, _P, _P],
     "et_matvec_f32": [_I, _IP, _P, _P, _P, _P, _P, _P],
     "et_matvec_f32_mfma": [_I, _IP, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P, _P, _P],
+    "et_residual_f64": [_I, _IP, _P, _P, _P, _P, _P, _P, _P],
+    "et_residual_f32": [_I, _IP, _P, _P, _P, _P, _P, _P, _P],
+    "et_jacobi_f64": [_I, _IP, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P, _P],
+    "et_jacobi_f32": [_I, _IP, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P, _P],
     "et_diagonal_f64": [_I, _IP, _P, _P, _P, _P, _P],
     "et_elem_energy_f64": [_I, _IP, _P, _P, _P, _P],
     "et_sensitivity_f64": [_I, _IP, _P, _P, _P, _D, _D, _P, _P],

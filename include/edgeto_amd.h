@@ -46,12 +46,30 @@ const char* et_last_error(void);
 /* Library version string. */
 const char* et_version(void);
 
-/* Launch-shape tuning knobs (process-wide; read by subsequent launches).
- * ET_TUNE_TILED_JC: y planes marched per workgroup by the fine-grid operator.
- * ET_TUNE_MFMA_JC: the same for the matrix-core FP32 operator.
- * ET_TUNE_MIXED_MFMA: 1 = mixed-precision V-cycles use the matrix-core fine-grid
- * operator; 0 (default) = FP32 vector FMAs.  Read by et_mg_create. */
-enum { ET_TUNE_TILED_JC = 0, ET_TUNE_MFMA_JC = 1, ET_TUNE_MIXED_MFMA = 2 };
+/* Launch-shape and kernel-choice knobs (process-wide; read by subsequent launches).
+ * ET_TUNE_TILED_JC: y planes marched per workgroup by the LDS-tiled fine-grid operator.
+ * ET_TUNE_MFMA_JC: the same for the FP16-split matrix-core FP32 operator.
+ * ET_TUNE_MIXED_MFMA: 1 = mixed-precision V-cycles use the FP16-split operator
+ *   (default 0).  Read by et_mg_create.
+ * ET_TUNE_MX_F32 / ET_TUNE_MX_F64: 1 = the fine-grid operator of that precision runs
+ *   on the matrix cores at full precision (FP32 / FP64 MFMA), 0 = LDS-tiled vector FMAs.
+ *   Read by et_mg_create and by each et_matvec call.
+ * ET_TUNE_MX_WGS_F32 / ET_TUNE_MX_WGS_F64: workgroups per compute unit launched by the
+ *   matrix-core operator (persistent grid).
+ * ET_TUNE_WHT: 1 (default) = the fine-grid operator (both precisions) runs in the corner
+ *   Walsh-Hadamard basis (45 FMAs per element); takes precedence over ET_TUNE_MX_*.
+ * ET_TUNE_WHT_WGS: workgroups per compute unit launched by that operator. */
+enum {
+  ET_TUNE_TILED_JC = 0,
+  ET_TUNE_MFMA_JC = 1,
+  ET_TUNE_MIXED_MFMA = 2,
+  ET_TUNE_MX_F32 = 3,
+  ET_TUNE_MX_F64 = 4,
+  ET_TUNE_MX_WGS_F32 = 5,
+  ET_TUNE_MX_WGS_F64 = 6,
+  ET_TUNE_WHT = 7,
+  ET_TUNE_WHT_WGS = 8
+};
 int et_tune_set(int param, int value);
 int et_tune_get(int param);
 
@@ -78,6 +96,25 @@ int et_matvec_f32(int dim, const int* nel, const float* K0, const float* E,
 int et_matvec_f32_mfma(int dim, const int* nel, const double* K0_host, const float* K0,
                        const float* E, const uint8_t* cons, const float* u, float* y,
                        hipStream_t stream);
+
+/* r = b - A_hat x (the V-cycle residual at the fine level; the reference forms it
+ * as b - A @ x with the matrix-free operator at level 0, solvers/_multigrid_cuda.py:143). */
+int et_residual_f64(int dim, const int* nel, const double* K0, const double* E,
+                    const uint8_t* cons, const double* x, const double* b, double* r,
+                    hipStream_t stream);
+int et_residual_f32(int dim, const int* nel, const float* K0, const float* E,
+                    const uint8_t* cons, const float* x, const float* b, float* r,
+                    hipStream_t stream);
+
+/* One damped Jacobi sweep, y = x + omega * Dinv * (b - A_hat x), fused in one pass.
+ * Replaces CuMultiGrid._jacobi_smoother (solvers/_multigrid_cuda.py:59-62) at the fine
+ * level (x + omega * (b - A @ x) / D).  Dinv is the inverse of diag(A_hat). */
+int et_jacobi_f64(int dim, const int* nel, const double* K0, const double* E,
+                  const uint8_t* cons, const double* x, const double* b, const double* Dinv,
+                  double omega, double* y, hipStream_t stream);
+int et_jacobi_f32(int dim, const int* nel, const float* K0, const float* E,
+                  const uint8_t* cons, const float* x, const float* b, const float* Dinv,
+                  double omega, float* y, hipStream_t stream);
 
 /* diag(A_hat); constrained entries are 1.  Replaces
  * get_diagonal_node_basis_cuda_kernel (core/_templated_cuda.cu:1068-1113). */

@@ -38,9 +38,29 @@ def _oracle_problem(nel, nu, rmin, clamp, load, fy):
     return prob
 
 
+# fine-grid operator variants: (ET_TUNE_WHT, ET_TUNE_MX_F32, ET_TUNE_MX_F64); None = library default
+_OPS = {"default": None, "tiled": (0, 0, 0), "matrix": (0, 1, 1), "wht": (1, 0, 0)}
+_KNOBS = (7, 3, 4)
+
+
+@pytest.fixture
+def fine_ops(request):
+    from edgeto_amd import _lib as L
+    L.load()
+    knobs = _OPS[request.param]
+    old = [L.lib().et_tune_get(k) for k in _KNOBS]
+    if knobs is not None:
+        for k, v in zip(_KNOBS, knobs):
+            L.call("et_tune_set", k, v)
+    yield request.param
+    for k, v in zip(_KNOBS, old):
+        L.call("et_tune_set", k, v)
+
+
+@pytest.mark.parametrize("fine_ops", ["default", "tiled", "matrix", "wht"], indirect=True)
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("nel,n_level", [((16, 8, 8), 2), ((24, 12, 12), 2)])
-def test_solve_matches_direct(precision, nel, n_level):
+def test_solve_matches_direct(precision, nel, n_level, fine_ops):
     ET = _edgeto()
     mesh, kernel, filt, clamp, load = _cantilever_3d(ET, nel)
     rng = np.random.default_rng(0)
