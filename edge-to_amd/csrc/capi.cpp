@@ -6,6 +6,7 @@
 #include <string>
 
 #include "blas.hpp"
+#include "coarse.hpp"
 #include "common.hpp"
 #include "galerkin_elem.hpp"
 #include "solver.hpp"
@@ -77,6 +78,11 @@ int et_tune_set(int param, int value) {
     } else if (param == ET_TUNE_MX_WGS_F32 || param == ET_TUNE_MX_WGS_F64) {
       ET_REQUIRE(value >= 1 && value <= 8, "et_tune_set: workgroups per CU must be in [1, 8]");
       et::set_mx_wgs(param == ET_TUNE_MX_WGS_F64, value);
+    } else if (param == ET_TUNE_WHT_THREADS_F32 || param == ET_TUNE_WHT_THREADS_F64) {
+      ET_REQUIRE(value == 512 || value == 1024, "et_tune_set: workgroup size must be 512 or 1024");
+      et::set_wht_threads(param == ET_TUNE_WHT_THREADS_F64, value);
+    } else if (param == ET_TUNE_COARSE_LIB) {
+      et::set_coarse_lib(value);
     } else if (param == ET_TUNE_WHT) {
       et::set_wht(value);
     } else if (param == ET_TUNE_WHT_WGS) {
@@ -97,6 +103,9 @@ int et_tune_get(int param) {
   if (param == ET_TUNE_MX_WGS_F32 || param == ET_TUNE_MX_WGS_F64)
     return et::get_mx_wgs(param == ET_TUNE_MX_WGS_F64);
   if (param == ET_TUNE_WHT) return et::get_wht();
+  if (param == ET_TUNE_COARSE_LIB) return et::get_coarse_lib();
+  if (param == ET_TUNE_WHT_THREADS_F32 || param == ET_TUNE_WHT_THREADS_F64)
+    return et::get_wht_threads(param == ET_TUNE_WHT_THREADS_F64);
   if (param == ET_TUNE_WHT_WGS) return et::get_wht_wgs();
   return -1;
 }

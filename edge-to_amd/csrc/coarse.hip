@@ -1,9 +1,13 @@
 // Coarsest-grid direct solve in FP64: the coarsest Galerkin stencil is
-// expanded to a dense SPD matrix, factored by a blocked right-looking
-// Cholesky (64x64 tiles in LDS), and its inverse is formed as
-// A^-1 = L^-T L^-1 so that every V-cycle's coarse solve is one dense
-// symmetric matrix-vector product (one wavefront per row, coalesced).
+// expanded to a dense SPD matrix, Cholesky-factored and inverted once per
+// solve, so that every V-cycle's coarse solve is one dense symmetric
+// matrix-vector product (one wavefront per row, coalesced).  Two factor
+// paths: rocSOLVER's potrf + potri (default), or the in-house blocked
+// right-looking Cholesky (64x64 tiles in LDS) with A^-1 = L^-T L^-1.
 // The matrix is padded to a multiple of 64 with an identity block.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "common.hpp"
 #include "coarse.hpp"
 
@@ -241,20 +245,40 @@ __global__ __launch_bounds__(256) void symv_kernel(const double* __restrict__ Ai
   if (lane == 0) x[row] = (TO)s;
 }
 
+// full symmetric matrix from the column-major lower triangle rocSOLVER leaves
+// (the row-major upper triangle): A[i][j] = A[j][i] for j < i
+__global__ __launch_bounds__(256) void mirror_upper_kernel(double* A, int ld, int n) {
+  const int i = blockIdx.x;
+  const double* src = A + i;  // column i of the row-major matrix: A[j][i] at src[j * ld]
+  for (int j = threadIdx.x; j < i && i < n; j += blockDim.x) A[(int64_t)i * ld + j] = src[(int64_t)j * ld];
+}
+
+static int g_coarse_lib = 1;
+void set_coarse_lib(int on) { g_coarse_lib = on ? 1 : 0; }
+int get_coarse_lib() { return g_coarse_lib; }
+
 int CoarseSolver::init(int n_, hipStream_t s) {
   n = n_;
   ld = (n + NB - 1) / NB * NB;
   nbt = ld / NB;
+  use_lib = g_coarse_lib != 0;
   const size_t bytes = (size_t)ld * ld * sizeof(double);
   ET_CHECK_HIP(hipMalloc(&A, bytes));
-  ET_CHECK_HIP(hipMalloc(&Y, bytes));
-  ET_CHECK_HIP(hipMalloc(&info, sizeof(int)));
-  ET_CHECK_HIP(hipHostMalloc(&info_h, sizeof(int)));
+  if (!use_lib) ET_CHECK_HIP(hipMalloc(&Y, bytes));
+  ET_CHECK_HIP(hipMalloc(&info, 2 * sizeof(int)));
+  ET_CHECK_HIP(hipHostMalloc(&info_h, 2 * sizeof(int)));
+  if (use_lib) {
+    rocblas_handle h = nullptr;
+    ET_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, "rocblas_create_handle failed");
+    blas = h;
+  }
   (void)s;
   return ET_OK;
 }
 
 void CoarseSolver::release() {
+  if (blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(blas));
+  blas = nullptr;
   if (A) (void)hipFree(A);
   if (Y) (void)hipFree(Y);
   if (info) (void)hipFree(info);
@@ -267,8 +291,8 @@ void CoarseSolver::release() {
 int CoarseSolver::factor(const Grid& g, int dof, const double* S, hipStream_t s) {
   const size_t bytes = (size_t)ld * ld * sizeof(double);
   ET_CHECK_HIP(hipMemsetAsync(A, 0, bytes, s));
-  ET_CHECK_HIP(hipMemsetAsync(Y, 0, bytes, s));
-  ET_CHECK_HIP(hipMemsetAsync(info, 0, sizeof(int), s));
+  if (Y) ET_CHECK_HIP(hipMemsetAsync(Y, 0, bytes, s));
+  ET_CHECK_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int), s));
   const int NO = g.dim == 3 ? 27 : 9;
   const unsigned grid = grid_for(g.n_nodes() * NO);
   if (g.dim == 3)
@@ -279,6 +303,18 @@ int CoarseSolver::factor(const Grid& g, int dof, const double* S, hipStream_t s)
   if (ld > n) {
     pad_identity_kernel<<<grid_for(ld - n), kBlock, 0, s>>>(n, ld, A);
     ET_CHECK_LAUNCH();
+  }
+  if (use_lib) {
+    rocblas_handle h = static_cast<rocblas_handle>(blas);
+    ET_REQUIRE(rocblas_set_stream(h, s) == rocblas_status_success, "rocblas_set_stream failed");
+    ET_REQUIRE(rocsolver_dpotrf(h, rocblas_fill_lower, ld, A, ld, info) == rocblas_status_success,
+               "rocsolver_dpotrf failed");
+    ET_REQUIRE(rocsolver_dpotri(h, rocblas_fill_lower, ld, A, ld, info + 1) == rocblas_status_success,
+               "rocsolver_dpotri failed");
+    mirror_upper_kernel<<<ld, 256, 0, s>>>(A, ld, ld);
+    ET_CHECK_LAUNCH();
+    ET_CHECK_HIP(hipMemcpyAsync(info_h, info, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    return ET_OK;
   }
   for (int kb = 0; kb < nbt; ++kb) {
     potrf_diag_kernel<<<1, 256, 0, s>>>(A, ld, kb, info);
@@ -299,15 +335,18 @@ int CoarseSolver::factor(const Grid& g, int dof, const double* S, hipStream_t s)
   // A <- Y^T Y = A_coarse^-1
   yty_kernel<<<nbt * (nbt + 1) / 2, 256, 0, s>>>(Y, A, ld, nbt);
   ET_CHECK_LAUNCH();
-  ET_CHECK_HIP(hipMemcpyAsync(info_h, info, sizeof(int), hipMemcpyDeviceToHost, s));
+  ET_CHECK_HIP(hipMemcpyAsync(info_h, info, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   return ET_OK;
 }
 
 int CoarseSolver::check_info(hipStream_t s) {
   ET_CHECK_HIP(hipStreamSynchronize(s));
-  if (*info_h != 0) {
-    set_error("coarse-grid Cholesky: matrix not positive definite at row " +
-              std::to_string(*info_h - 1));
+  if (info_h[0] != 0) {
+    set_error("coarse-grid Cholesky: matrix not positive definite at row " + std::to_string(info_h[0] - 1));
+    return ET_ENONFINITE;
+  }
+  if (info_h[1] != 0) {
+    set_error("coarse-grid inverse: singular factor at row " + std::to_string(info_h[1] - 1));
     return ET_ENONFINITE;
   }
   return ET_OK;

@@ -1,4 +1,4 @@
-// Dense FP64 coarsest-grid solver (blocked Cholesky + explicit inverse).
+// Dense FP64 coarsest-grid solver (Cholesky + explicit inverse).
 #pragma once
 #include "common.hpp"
 
@@ -10,8 +10,10 @@ struct CoarseSolver {
   int nbt = 0;  // tiles per side
   double* A = nullptr;  // after factor(): A^-1 (ld x ld)
   double* Y = nullptr;  // L^-1 workspace
-  int* info = nullptr;
-  int* info_h = nullptr;
+  int* info = nullptr;    // [0] factor, [1] inverse (device)
+  int* info_h = nullptr;  // pinned host copy
+  bool use_lib = true;    // rocSOLVER potrf + potri, else the in-house kernels
+  void* blas = nullptr;   // rocblas_handle
 
   int init(int n, hipStream_t s);
   void release();
@@ -22,5 +24,9 @@ struct CoarseSolver {
   template <typename TI, typename TO>
   int solve(const TI* b, TO* x, hipStream_t s) const;
 };
+
+// Factor path of coarse solvers created afterwards: 1 = rocSOLVER (default).
+void set_coarse_lib(int on);
+int get_coarse_lib();
 
 }  // namespace et

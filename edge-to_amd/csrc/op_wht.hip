@@ -43,12 +43,21 @@
 
 namespace et {
 
-constexpr int kWThreads = 512;
-constexpr int kWSlots = 2;   // elements per thread per layer (16 slices of 64)
-constexpr int kWStage = 3;   // staged nodes per thread per plane ((TC+1)*nz <= 1536)
-constexpr int kWEStage = 2;  // staged E values per thread per layer (TC*nelz <= 1024)
-constexpr int kWNodes = 2;   // output nodes per thread per plane ((TC-1)*nz <= 1024)
-constexpr int kWG = 63;      // transformed blocks M = 1..7, 3 x 3 each
+// Per-workgroup capacities (independent of the workgroup size): 16 slices of
+// 64 elements per layer, (TC+1)*nz <= 1536 staged nodes, TC*nelz <= 1024 E
+// values, (TC-1)*nz <= 1024 output nodes.
+constexpr int kWStageCap = 1536, kWECap = 1024, kWNodeCap = 1024;
+constexpr int kWG = 63;  // transformed blocks M = 1..7, 3 x 3 each
+
+// workgroup of NT threads (512 or 1024)
+template <int NT>
+struct WCfg {
+  static constexpr int waves = NT / 64;
+  static constexpr int slots = 16 / waves;                     // elements per thread per layer
+  static constexpr int stage = (kWStageCap + NT - 1) / NT;     // staged nodes per thread
+  static constexpr int estage = kWECap / NT;                   // staged E per thread
+  static constexpr int nodes = kWNodeCap / NT;                 // output nodes per thread
+};
 
 // structural non-zeros of G[M][c][d] (M = 1..7)
 __host__ __device__ constexpr bool wht_nz(int M, int c, int d) {
@@ -104,8 +113,8 @@ __device__ inline void wht_face(const T u00, const T u10, const T u01, const T u
   F[3] = a1 - b1;
 }
 
-template <typename T, int MODE>
-__global__ __launch_bounds__(kWThreads) void op0_wht_kernel(const MxArgs A, const T* __restrict__ K0,
+template <typename T, int MODE, int NT>
+__global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __restrict__ K0,
                                                             const T* __restrict__ E,
                                                             const uint8_t* __restrict__ cons,
                                                             const T* __restrict__ x,
@@ -128,6 +137,11 @@ __global__ __launch_bounds__(kWThreads) void op0_wht_kernel(const MxArgs A, cons
   T* sg = se + tc * nelz;                 // G, 63 entries
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t plane = (int64_t)nx * nz;
+  constexpr int kWThreads = NT;
+  constexpr int kWSlots = WCfg<NT>::slots;
+  constexpr int kWStage = WCfg<NT>::stage;
+  constexpr int kWEStage = WCfg<NT>::estage;
+  constexpr int kWNodes = WCfg<NT>::nodes;
 
   // G[M][c][d] from K0 (corner order of K0 is the H8 local order)
   if (tid < kWG) {
@@ -161,7 +175,7 @@ __global__ __launch_bounds__(kWThreads) void op0_wht_kernel(const MxArgs A, cons
   int scl[kWSlots], sez[kWSlots];
 #pragma unroll
   for (int m = 0; m < kWSlots; ++m) {
-    const int s = wave + 8 * m;
+    const int s = wave + WCfg<NT>::waves * m;
     scl[m] = s / spc;
     sez[m] = 64 * (s % spc) + lane;
   }
@@ -419,6 +433,10 @@ __global__ __launch_bounds__(kWThreads) void op0_wht_kernel(const MxArgs A, cons
 
 static int g_wht_enable = 1;
 static int g_wht_wgs = 1;
+static int g_wht_nt[2] = {1024, 512};  // workgroup size [float, double]
+
+void set_wht_threads(int is_double, int nt) { g_wht_nt[is_double ? 1 : 0] = nt == 1024 ? 1024 : 512; }
+int get_wht_threads(int is_double) { return g_wht_nt[is_double ? 1 : 0]; }
 
 void set_wht(int on) { g_wht_enable = on ? 1 : 0; }
 int get_wht() { return g_wht_enable; }
@@ -430,8 +448,8 @@ constexpr size_t kWMaxLds = 160 * 1024;
 bool wht_supported(const Grid& g, size_t elem_bytes) {
   if (g.dim != 3 || g.ez > 128) return false;
   const int tc = wht_tc(g);
-  return (tc + 1) * g.nz() <= kWThreads * kWStage && tc * g.ez <= kWThreads * kWEStage &&
-         (tc - 1) * g.nz() <= kWThreads * kWNodes && wht_lds_values(g) * elem_bytes <= kWMaxLds;
+  return (tc + 1) * g.nz() <= kWStageCap && tc * g.ez <= kWECap && (tc - 1) * g.nz() <= kWNodeCap &&
+         wht_lds_values(g) * elem_bytes <= kWMaxLds;
 }
 
 static int wht_device_cus() {
@@ -445,17 +463,27 @@ static int wht_device_cus() {
   return cus;
 }
 
+template <typename T, int MODE, int NT>
+static void launch_wht_nt(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const T* K0, const T* E,
+                          const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
+                          double wscale, const T* xadd, T* y) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&op0_wht_kernel<T, MODE, NT>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWMaxLds);
+    attr_set = true;
+  }
+  op0_wht_kernel<T, MODE, NT><<<n_wg, NT, lds, s>>>(A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y);
+}
+
 template <typename T, int MODE>
 static void launch_wht(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const T* K0, const T* E,
                        const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
                        double wscale, const T* xadd, T* y) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&op0_wht_kernel<T, MODE>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWMaxLds);
-    attr_set = true;
-  }
-  op0_wht_kernel<T, MODE><<<n_wg, kWThreads, lds, s>>>(A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y);
+  if (g_wht_nt[sizeof(T) == 8 ? 1 : 0] == 1024)
+    launch_wht_nt<T, MODE, 1024>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y);
+  else
+    launch_wht_nt<T, MODE, 512>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y);
 }
 
 template <typename T>

@@ -41,6 +41,8 @@ void set_wht(int on);
 int get_wht();
 void set_wht_wgs(int n);
 int get_wht_wgs();
+void set_wht_threads(int is_double, int nt);
+int get_wht_threads(int is_double);
 // Fine-grid dispatch: the transformed-basis operator where enabled and
 // supported, else the matrix-core one where enabled, else the LDS-tiled one.
 template <typename T>

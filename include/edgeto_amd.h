@@ -58,7 +58,11 @@ const char* et_version(void);
  *   matrix-core operator (persistent grid).
  * ET_TUNE_WHT: 1 (default) = the fine-grid operator (both precisions) runs in the corner
  *   Walsh-Hadamard basis (45 FMAs per element); takes precedence over ET_TUNE_MX_*.
- * ET_TUNE_WHT_WGS: workgroups per compute unit launched by that operator. */
+ * ET_TUNE_WHT_WGS: workgroups per compute unit launched by that operator.
+ * ET_TUNE_COARSE_LIB: 1 (default) = the coarsest grid is factored and inverted with
+ *   rocSOLVER (potrf + potri); 0 = in-house blocked Cholesky kernels.  Read by et_mg_create.
+ * ET_TUNE_WHT_THREADS_F32 / _F64: workgroup size (512 or 1024) of the transformed-basis
+ *   operator. */
 enum {
   ET_TUNE_TILED_JC = 0,
   ET_TUNE_MFMA_JC = 1,
@@ -68,7 +72,10 @@ enum {
   ET_TUNE_MX_WGS_F32 = 5,
   ET_TUNE_MX_WGS_F64 = 6,
   ET_TUNE_WHT = 7,
-  ET_TUNE_WHT_WGS = 8
+  ET_TUNE_WHT_WGS = 8,
+  ET_TUNE_COARSE_LIB = 9,
+  ET_TUNE_WHT_THREADS_F32 = 10,
+  ET_TUNE_WHT_THREADS_F64 = 11
 };
 int et_tune_set(int param, int value);
 int et_tune_get(int param);

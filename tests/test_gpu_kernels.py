@@ -313,24 +313,27 @@ MX_MESHES = [(6, 4, 4), (8, 6, 20), (5, 3, 33), (12, 5, 16), (20, 3, 128), (9, 1
 
 WHT_MESHES = MX_MESHES + [(7, 4, 64), (5, 3, 65), (30, 4, 100), (4, 2, 1)]
 
-# fine-operator variants by knob values (ET_TUNE_WHT, ET_TUNE_MX_F32, ET_TUNE_MX_F64)
-_FINE = {"tiled": (0, 0, 0), "matrix": (0, 1, 1), "wht": (1, 0, 0)}
+# fine-operator variants by knob values
+# (ET_TUNE_WHT, ET_TUNE_MX_F32, ET_TUNE_MX_F64, ET_TUNE_WHT_THREADS_F32, ET_TUNE_WHT_THREADS_F64)
+_KN = (7, 3, 4, 10, 11)
+_FINE = {"tiled": (0, 0, 0, 1024, 512), "matrix": (0, 1, 1, 1024, 512), "wht": (1, 0, 0, 1024, 512),
+         "wht_alt": (1, 0, 0, 512, 1024)}
 
 
 def _matvec_variant(L, nel, prec, variant, seed):
-    old = [L.lib().et_tune_get(k) for k in (7, 3, 4)]
+    old = [L.lib().et_tune_get(k) for k in _KN]
     p = _problem(nel, seed=seed)
     ref = oops.matvec(p["K0"], p["el"], p["E"], p["u"], p["cons"])
     dt = torch.float32 if prec == "f32" else torch.float64
     y = torch.empty(p["n_dof"], dtype=dt, device="cuda")
-    for k, v in zip((7, 3, 4), _FINE[variant]):
+    for k, v in zip(_KN, _FINE[variant]):
         L.call("et_tune_set", k, v)
     try:
         L.call("et_matvec_" + prec, 3, L.nel_arg(nel), L.ptr(dev(p["K0"], dt)), L.ptr(dev(p["E"], dt)),
                L.ptr(dev(p["cons"].astype(np.uint8))), L.ptr(dev(p["u"], dt)), L.ptr(y), L.stream())
         torch.cuda.synchronize()
     finally:
-        for k, v in zip((7, 3, 4), old):
+        for k, v in zip(_KN, old):
             L.call("et_tune_set", k, v)
     return y.cpu().numpy(), ref
 
@@ -347,10 +350,12 @@ def test_matvec_matrix_core(nel, prec):
 
 @pytest.mark.parametrize("nel", WHT_MESHES)
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_matvec_transformed_basis(nel, prec):
-    """Walsh-Hadamard-basis fine operator: both tile widths (nelz <= 64 and <= 128), the 64 and
-    128 seams, a single z layer, and more (x tile, plane) units than persistent workgroups."""
-    y, ref = _matvec_variant(_lib(), nel, prec, "wht", 22)
+@pytest.mark.parametrize("variant", ["wht", "wht_alt"])
+def test_matvec_transformed_basis(nel, prec, variant):
+    """Walsh-Hadamard-basis fine operator (both workgroup sizes): both tile widths (nelz <= 64
+    and <= 128), the 64 and 128 seams, a single z layer, and more (x tile, plane) units than
+    persistent workgroups."""
+    y, ref = _matvec_variant(_lib(), nel, prec, variant, 22)
     assert rel(y, ref) < (1e-5 if prec == "f32" else 1e-12)
 
 
@@ -359,13 +364,13 @@ SWEEP_MESHES = [(6, 4, 4), (7, 5), (20, 3, 128), (48, 20, 100), (40, 30, 40)]
 
 @pytest.mark.parametrize("nel", SWEEP_MESHES)
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-@pytest.mark.parametrize("variant", ["tiled", "matrix", "wht"])
+@pytest.mark.parametrize("variant", ["tiled", "matrix", "wht", "wht_alt"])
 def test_residual_and_jacobi(nel, prec, variant):
     """Fused residual and damped-Jacobi modes of every fine-grid operator variant against the
     oracle (r = b - A x, y = x + w D^-1 r), on meshes with many y planes per workgroup."""
     L = _lib()
     dim = len(nel)
-    old = [L.lib().et_tune_get(k) for k in (7, 3, 4)]
+    old = [L.lib().et_tune_get(k) for k in _KN]
     p = _problem(nel, seed=23)
     rng = np.random.default_rng(5)
     b = rng.standard_normal(p["n_dof"])
@@ -381,7 +386,7 @@ def test_residual_and_jacobi(nel, prec, variant):
     bt = dev(b, dt)
     r = torch.empty(p["n_dof"], dtype=dt, device="cuda")
     y = torch.empty(p["n_dof"], dtype=dt, device="cuda")
-    for k, v in zip((7, 3, 4), _FINE[variant]):
+    for k, v in zip(_KN, _FINE[variant]):
         L.call("et_tune_set", k, v)
     try:
         L.call("et_residual_" + prec, dim, L.nel_arg(nel), *args, L.ptr(x), L.ptr(bt), L.ptr(r), L.stream())
@@ -389,7 +394,7 @@ def test_residual_and_jacobi(nel, prec, variant):
                L.ptr(dev(Dinv, dt)), omega, L.ptr(y), L.stream())
         torch.cuda.synchronize()
     finally:
-        for k, v in zip((7, 3, 4), old):
+        for k, v in zip(_KN, old):
             L.call("et_tune_set", k, v)
     assert rel(r.cpu().numpy(), r_ref) < tol
     assert rel(y.cpu().numpy(), y_ref) < tol
