@@ -38,44 +38,69 @@ __device__ __forceinline__ double tw_of(const Grid& gc, int i, int j, int k) {
 }
 
 // ------------------------------------------------------------ restriction ---
+// One thread per coarse node, all DOF components at once; 32-bit node
+// arithmetic (node counts are below 2^31 by construction of the Grid).
 
-template <typename T, int DIM>
-__global__ __launch_bounds__(256) void restrict_kernel(const Grid gf, const Grid gc, int dof,
+template <int DIM>
+__device__ __forceinline__ void node_coords32(int n, int nx, int nz, int& i, int& j, int& k) {
+  if (DIM == 3) {
+    k = n % nz;
+    n /= nz;
+  } else {
+    k = 0;
+  }
+  i = n % nx;
+  j = n / nx;
+}
+
+template <typename T, int DIM, int DOF>
+__global__ __launch_bounds__(256) void restrict_kernel(const Grid gf, const Grid gc,
                                                        const T* __restrict__ vf,
                                                        T* __restrict__ vc) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= gc.n_nodes()) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int)gc.n_nodes()) return;
+  const int fnx = gf.nx(), fny = gf.ny(), fnz = gf.nz();
   int ci, cj, ck;
-  node_coords<DIM>(gc, c, ci, cj, ck);
-  const T tw = (T)tw_of<DIM>(gc, ci, cj, ck);
-  for (int d = 0; d < dof; ++d) {
-    T v = 0;
+  node_coords32<DIM>(c, gc.nx(), gc.nz(), ci, cj, ck);
+  T acc[DOF];
+#pragma unroll
+  for (int d = 0; d < DOF; ++d) acc[d] = T(0);
+#pragma unroll
+  for (int dj = -1; dj <= 1; ++dj) {
+    const int fj = 2 * cj + dj;
+    if (fj < 0 || fj >= fny) continue;
+#pragma unroll
     for (int di = -1; di <= 1; ++di) {
       const int fi = 2 * ci + di;
-      if (fi < 0 || fi >= gf.nx()) continue;
-      for (int dj = -1; dj <= 1; ++dj) {
-        const int fj = 2 * cj + dj;
-        if (fj < 0 || fj >= gf.ny()) continue;
-        for (int dk = (DIM == 3 ? -1 : 0); dk <= (DIM == 3 ? 1 : 0); ++dk) {
-          const int fk = 2 * ck + dk;
-          if (DIM == 3 && (fk < 0 || fk >= gf.nz())) continue;
-          const T wgt = (T)(pw(di) * pw(dj) * pw(dk));
-          v += wgt * vf[node_index<DIM>(gf, fi, fj, fk) * dof + d];
-        }
+      if (fi < 0 || fi >= fnx) continue;
+#pragma unroll
+      for (int dk = (DIM == 3 ? -1 : 0); dk <= (DIM == 3 ? 1 : 0); ++dk) {
+        const int fk = 2 * ck + dk;
+        if (DIM == 3 && (fk < 0 || fk >= fnz)) continue;
+        const T wgt = (T)(pw(di) * pw(dj) * pw(dk));
+        const int f = DIM == 3 ? (fj * fnx + fi) * fnz + fk : fj * fnx + fi;
+#pragma unroll
+        for (int d = 0; d < DOF; ++d) acc[d] += wgt * vf[(int64_t)f * DOF + d];
       }
     }
-    vc[c * dof + d] = v / tw;
   }
+  const T inv = (T)(1.0 / tw_of<DIM>(gc, ci, cj, ck));
+#pragma unroll
+  for (int d = 0; d < DOF; ++d) vc[(int64_t)c * DOF + d] = acc[d] * inv;
 }
 
 template <typename T>
 int restrict_op(const Grid& gf, int dof, const T* vf, T* vc, hipStream_t s) {
+  ET_REQUIRE(dof >= 1 && dof <= 3, "restriction: dof must be 1, 2 or 3");
   const Grid gc = gf.coarse();
   const unsigned grid = grid_for(gc.n_nodes());
-  if (gf.dim == 3)
-    restrict_kernel<T, 3><<<grid, kBlock, 0, s>>>(gf, gc, dof, vf, vc);
-  else
-    restrict_kernel<T, 2><<<grid, kBlock, 0, s>>>(gf, gc, dof, vf, vc);
+#define ET_R(DIM, DOF) restrict_kernel<T, DIM, DOF><<<grid, kBlock, 0, s>>>(gf, gc, vf, vc)
+  if (gf.dim == 3) {
+    if (dof == 3) ET_R(3, 3); else if (dof == 2) ET_R(3, 2); else ET_R(3, 1);
+  } else {
+    if (dof == 3) ET_R(2, 3); else if (dof == 2) ET_R(2, 2); else ET_R(2, 1);
+  }
+#undef ET_R
   ET_CHECK_LAUNCH();
   return ET_OK;
 }
@@ -83,43 +108,54 @@ template int restrict_op<double>(const Grid&, int, const double*, double*, hipSt
 template int restrict_op<float>(const Grid&, int, const float*, float*, hipStream_t);
 
 // ----------------------------------------------------------- prolongation ---
+// One thread per fine node: average of the (1, 2, 4 or 8) in-range coarse
+// nodes of its cell, each divided by its total weight tw.
 
-template <typename T, int DIM>
-__global__ __launch_bounds__(256) void prolong_kernel(const Grid gf, const Grid gc, int dof,
+template <typename T, int DIM, int DOF>
+__global__ __launch_bounds__(256) void prolong_kernel(const Grid gf, const Grid gc,
                                                       const T* __restrict__ vc,
                                                       T* __restrict__ vf) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= gf.n_nodes()) return;
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= (int)gf.n_nodes()) return;
+  const int cnx = gc.nx(), cny = gc.ny(), cnz = gc.nz();
   int i, j, k;
-  node_coords<DIM>(gf, f, i, j, k);
-  const int ci0 = i / 2, cj0 = j / 2, ck0 = k / 2;
-  for (int d = 0; d < dof; ++d) {
-    T v = 0;
-    int count = 0;
-    for (int ci = ci0; ci <= ci0 + (i % 2); ++ci) {
-      if (ci >= gc.nx()) continue;
-      for (int cj = cj0; cj <= cj0 + (j % 2); ++cj) {
-        if (cj >= gc.ny()) continue;
-        for (int ck = ck0; ck <= ck0 + (DIM == 3 ? (k % 2) : 0); ++ck) {
-          if (DIM == 3 && ck >= gc.nz()) continue;
-          const T tw = (T)tw_of<DIM>(gc, ci, cj, ck);
-          v += vc[node_index<DIM>(gc, ci, cj, ck) * dof + d] / tw;
-          ++count;
-        }
+  node_coords32<DIM>(f, gf.nx(), gf.nz(), i, j, k);
+  const int ci0 = i >> 1, cj0 = j >> 1, ck0 = k >> 1;
+  const int ni = (i & 1) && ci0 + 1 < cnx ? 2 : 1;
+  const int nj = (j & 1) && cj0 + 1 < cny ? 2 : 1;
+  const int nk = DIM == 3 && (k & 1) && ck0 + 1 < cnz ? 2 : 1;
+  T acc[DOF];
+#pragma unroll
+  for (int d = 0; d < DOF; ++d) acc[d] = T(0);
+  int count = 0;
+  for (int a = 0; a < nj; ++a)
+    for (int bq = 0; bq < ni; ++bq)
+      for (int cq = 0; cq < nk; ++cq) {
+        const int ci = ci0 + bq, cj = cj0 + a, ck = ck0 + cq;
+        if (ci >= cnx || cj >= cny || (DIM == 3 && ck >= cnz)) continue;
+        const T inv = (T)(1.0 / tw_of<DIM>(gc, ci, cj, ck));
+        const int c = DIM == 3 ? (cj * cnx + ci) * cnz + ck : cj * cnx + ci;
+#pragma unroll
+        for (int d = 0; d < DOF; ++d) acc[d] += vc[(int64_t)c * DOF + d] * inv;
+        ++count;
       }
-    }
-    vf[f * dof + d] = count > 0 ? v / (T)count : T(0);
-  }
+  const T sc = count > 0 ? T(1) / (T)count : T(0);
+#pragma unroll
+  for (int d = 0; d < DOF; ++d) vf[(int64_t)f * DOF + d] = acc[d] * sc;
 }
 
 template <typename T>
 int prolong_op(const Grid& gf, int dof, const T* vc, T* vf, hipStream_t s) {
+  ET_REQUIRE(dof >= 1 && dof <= 3, "prolongation: dof must be 1, 2 or 3");
   const Grid gc = gf.coarse();
   const unsigned grid = grid_for(gf.n_nodes());
-  if (gf.dim == 3)
-    prolong_kernel<T, 3><<<grid, kBlock, 0, s>>>(gf, gc, dof, vc, vf);
-  else
-    prolong_kernel<T, 2><<<grid, kBlock, 0, s>>>(gf, gc, dof, vc, vf);
+#define ET_P(DIM, DOF) prolong_kernel<T, DIM, DOF><<<grid, kBlock, 0, s>>>(gf, gc, vc, vf)
+  if (gf.dim == 3) {
+    if (dof == 3) ET_P(3, 3); else if (dof == 2) ET_P(3, 2); else ET_P(3, 1);
+  } else {
+    if (dof == 3) ET_P(2, 3); else if (dof == 2) ET_P(2, 2); else ET_P(2, 1);
+  }
+#undef ET_P
   ET_CHECK_LAUNCH();
   return ET_OK;
 }
