@@ -177,7 +177,21 @@ def run_batch(ET, n_problems, iters):
                                     "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)}}
 
 
-def roofline(prof, n_dof, n_el):
+def measured_traffic(kernel_class, workload):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by its streaming-kernel calibration
+    factor), or None when no summary for this kernel and workload is present."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{kernel_class}_traffic.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return (t["fetch_kib"] * 1024.0 * t["fetch_factor"] + t["write_kib"] * 1024.0 * t["write_factor"],
+            os.path.relpath(path, ROOT))
+
+
+def roofline(prof, n_dof, n_el, workload):
     """Dominant kernel class of the mixed leg and its achieved algorithmic bandwidth."""
     s = 4  # FP32 V-cycle vectors / E
     mask = n_dof  # one byte per DOF
@@ -192,10 +206,14 @@ def roofline(prof, n_dof, n_el):
     ms, count = cand[name]
     avg_s = ms / count * 1e-3
     achieved = bytes_per[name] / avg_s / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": name, "avg_us": round(avg_s * 1e6, 2), "launches": count,
-            "bytes_per_launch": bytes_per[name]}
+    tr = measured_traffic(name, workload)
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None if tr is None else round(tr[0]),
+           "kernel": name, "avg_us": round(avg_s * 1e6, 2), "launches": count,
+           "bytes_per_launch": bytes_per[name]}
+    if tr is not None:
+        out["traffic_source"] = tr[1]
+    return out
 
 
 def cpu_baseline(nel_metric, sample_nel=(96, 48, 48), n_level=3):
@@ -239,6 +257,8 @@ def main():
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c5"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--legs", default="mixed,fp64")
+    ap.add_argument("--validate", type=int, default=30,
+                    help="OC iterations of the mixed-vs-FP64 compliance gate (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); no multi-GPU path "
@@ -275,8 +295,18 @@ def main():
         out["speedup_vs_fp64"] = round(f["elapsed"] / main_leg["elapsed"], 3)
         out["compliance_rel_diff"] = abs(main_leg["comp"] - f["comp"]) / abs(f["comp"])
     if main_leg["prof"] is not None:
-        out["roofline"] = roofline(main_leg["prof"], main_leg["n_dof"], main_leg["n_el"])
+        out["roofline"] = roofline(main_leg["prof"], main_leg["n_dof"], main_leg["n_el"], args.workload)
         out["kernel_classes_ms"] = {k: round(v[0], 2) for k, v in main_leg["prof"].items()}
+    if args.validate > 0 and "fp64" in legs and "mixed" in legs:
+        # validity gate: both precisions from the same start for `validate` OC iterations
+        comp = {}
+        for leg in ("mixed", "fp64"):
+            print(f"[bench] validity run {leg}: {args.validate} OC iterations", file=sys.stderr, flush=True)
+            comp[leg] = run_leg(ET, wl, leg, args.validate, 0)["comp"]
+        rd = abs(comp["mixed"] - comp["fp64"]) / abs(comp["fp64"])
+        out["validity"] = {"oc_iterations": args.validate, "compliance_mixed": comp["mixed"],
+                           "compliance_fp64": comp["fp64"], "rel_diff": rd, "gate": 1e-6,
+                           "pass": bool(rd <= 1e-6)}
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle sample)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(wl["nel"])
