@@ -64,6 +64,11 @@ const char* et_version(void) { return "edgeto_amd 0.1 (gfx950)"; }
 
 int et_tune_set(int param, int value) {
   ET_GUARD({
+    if (param == ET_TUNE_ROW_STENCIL_NODES) {
+      ET_REQUIRE(value >= 0, "et_tune_set: node count must be >= 0");
+      et::set_row_stencil_nodes(value);
+      return ET_OK;
+    }
     ET_REQUIRE(value >= 0 && value <= 1024, "et_tune_set: value out of range");
     if (param == ET_TUNE_TILED_JC) {
       ET_REQUIRE(value >= 1, "et_tune_set: jc must be >= 1");
@@ -104,6 +109,7 @@ int et_tune_get(int param) {
     return et::get_mx_wgs(param == ET_TUNE_MX_WGS_F64);
   if (param == ET_TUNE_WHT) return et::get_wht();
   if (param == ET_TUNE_COARSE_LIB) return et::get_coarse_lib();
+  if (param == ET_TUNE_ROW_STENCIL_NODES) return (int)et::get_row_stencil_nodes();
   if (param == ET_TUNE_WHT_THREADS_F32 || param == ET_TUNE_WHT_THREADS_F64)
     return et::get_wht_threads(param == ET_TUNE_WHT_THREADS_F64);
   if (param == ET_TUNE_WHT_WGS) return et::get_wht_wgs();

@@ -168,6 +168,70 @@ __global__ __launch_bounds__(256) void opS_kernel(const Grid g, const T* __restr
   op_epilogue<T, DIM, MODE>(n, acc, x, b, Dinv, w, xadd, y);
 }
 
+// The same operator with one thread per (node, component): three times the
+// threads of opS_kernel, for the small coarse levels where one thread per node
+// leaves the GPU latency-bound.  Output component r = t / N of node n = t % N.
+template <typename T, int DIM, int MODE>
+__global__ __launch_bounds__(256) void opS_row_kernel(const Grid g, const T* __restrict__ S,
+                                                      const T* __restrict__ x, const T* __restrict__ b,
+                                                      const T* __restrict__ Dinv,
+                                                      const double* __restrict__ omega, double wscale,
+                                                      const T* __restrict__ xadd,
+                                                      T* __restrict__ y) {
+  constexpr int OC = DIM == 3 ? 13 : 4;
+  constexpr int NH = OC + 1;
+  const int N = (int)g.n_nodes();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * DIM) return;
+  const int r = t / N, n = t - r * N;
+  const int nz = g.nz(), nx = g.nx(), ny = g.ny();
+  const int k = DIM == 3 ? n % nz : 0;
+  const int ij = DIM == 3 ? n / nz : n;
+  const int i = ij % nx, j = ij / nx;
+  T acc = T(0);
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {
+    const int o = OC + h;
+    const int oi = DIM == 3 ? o / 9 - 1 : o / 3 - 1;
+    const int oj = DIM == 3 ? (o / 3) % 3 - 1 : o % 3 - 1;
+    const int ok = DIM == 3 ? o % 3 - 1 : 0;
+    const int off = DIM == 3 ? (oj * nx + oi) * nz + ok : oj * nx + oi;
+    {
+      const int mi = i + oi, mj = j + oj, mk = k + ok;
+      if (mi >= 0 && mi < nx && mj >= 0 && mj < ny && (DIM == 2 || (mk >= 0 && mk < nz))) {
+        const int m = n + off;
+#pragma unroll
+        for (int q = 0; q < DIM; ++q)
+          acc += S[((int64_t)(h * DIM + r) * DIM + q) * N + n] * x[(int64_t)m * DIM + q];
+      }
+    }
+    if (h > 0) {
+      const int mi = i - oi, mj = j - oj, mk = k - ok;
+      if (mi >= 0 && mi < nx && mj >= 0 && mj < ny && (DIM == 2 || (mk >= 0 && mk < nz))) {
+        const int m = n - off;
+#pragma unroll
+        for (int q = 0; q < DIM; ++q)
+          acc += S[((int64_t)(h * DIM + q) * DIM + r) * N + m] * x[(int64_t)m * DIM + q];
+      }
+    }
+  }
+  const int64_t d = (int64_t)n * DIM + r;
+  const T w = MODE >= OP_JAC ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
+  T v;
+  if (MODE == OP_Y)
+    v = acc;
+  else if (MODE == OP_RES)
+    v = b[d] - acc;
+  else
+    v = x[d] + w * Dinv[d] * (b[d] - acc);
+  if (MODE == OP_JAC_ADD) v += xadd[d];
+  y[d] = v;
+}
+
+static int64_t g_row_stencil_max_nodes = 300000;  // coarse levels up to this size use opS_row_kernel
+void set_row_stencil_nodes(int64_t n) { g_row_stencil_max_nodes = n; }
+int64_t get_row_stencil_nodes() { return g_row_stencil_max_nodes; }
+
 // y = omega*wscale * Dinv * b  (first Jacobi sweep from a zero guess)
 template <typename T>
 __global__ void scale_kernel(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ b,
@@ -335,8 +399,14 @@ static int op_launch(const MGLevel<T>& L, const MGShared& sh, int mode, const T*
 #define ET_OP0(DIM, MODE)                                                                   \
   op0_kernel<T, DIM, MODE><<<grid, kBlock, 0, s>>>(g, sh.K0<T>(), sh.E<T>(), sh.cons, x, b, \
                                                    L.Dinv, omega, wscale, xadd, y)
-#define ET_OPS(DIM, MODE) \
-  opS_kernel<T, DIM, MODE><<<grid, kBlock, 0, s>>>(g, L.S, x, b, L.Dinv, omega, wscale, xadd, y)
+#define ET_OPS(DIM, MODE)                                                                            \
+  do {                                                                                               \
+    if (g.n_nodes() <= g_row_stencil_max_nodes)                                                      \
+      opS_row_kernel<T, DIM, MODE><<<grid_for(g.n_nodes() * DIM), kBlock, 0, s>>>(g, L.S, x, b, L.Dinv, \
+                                                                                  omega, wscale, xadd, y); \
+    else                                                                                             \
+      opS_kernel<T, DIM, MODE><<<grid, kBlock, 0, s>>>(g, L.S, x, b, L.Dinv, omega, wscale, xadd, y); \
+  } while (0)
 #define ET_DISPATCH(M)                \
   do {                                \
     if (L.level == 0) {               \

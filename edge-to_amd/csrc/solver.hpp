@@ -143,6 +143,9 @@ struct MGSolver {
   int apply_A64(const double* x, double* y, int mode, const double* b);
 };
 
+void set_row_stencil_nodes(int64_t n);
+int64_t get_row_stencil_nodes();
+
 }  // namespace et
 
 struct et_mg {

@@ -62,7 +62,9 @@ const char* et_version(void);
  * ET_TUNE_COARSE_LIB: 1 (default) = the coarsest grid is factored and inverted with
  *   rocSOLVER (potrf + potri); 0 = in-house blocked Cholesky kernels.  Read by et_mg_create.
  * ET_TUNE_WHT_THREADS_F32 / _F64: workgroup size (512 or 1024) of the transformed-basis
- *   operator. */
+ *   operator.
+ * ET_TUNE_ROW_STENCIL_NODES: coarse levels with at most this many nodes apply their
+ *   stencil with one thread per (node, component) instead of one per node. */
 enum {
   ET_TUNE_TILED_JC = 0,
   ET_TUNE_MFMA_JC = 1,
@@ -75,7 +77,8 @@ enum {
   ET_TUNE_WHT_WGS = 8,
   ET_TUNE_COARSE_LIB = 9,
   ET_TUNE_WHT_THREADS_F32 = 10,
-  ET_TUNE_WHT_THREADS_F64 = 11
+  ET_TUNE_WHT_THREADS_F64 = 11,
+  ET_TUNE_ROW_STENCIL_NODES = 12
 };
 int et_tune_set(int param, int value);
 int et_tune_get(int param);

@@ -39,8 +39,9 @@ def _oracle_problem(nel, nu, rmin, clamp, load, fy):
 
 
 # fine-grid operator variants: (ET_TUNE_WHT, ET_TUNE_MX_F32, ET_TUNE_MX_F64); None = library default
-_OPS = {"default": None, "tiled": (0, 0, 0, 0), "matrix": (0, 1, 1, 1), "wht": (1, 0, 0, 1)}
-_KNOBS = (7, 3, 4, 9)  # ET_TUNE_WHT, ET_TUNE_MX_F32, ET_TUNE_MX_F64, ET_TUNE_COARSE_LIB
+_OPS = {"default": None, "tiled": (0, 0, 0, 0, 0), "matrix": (0, 1, 1, 1, 1 << 30), "wht": (1, 0, 0, 1, 300000)}
+# ET_TUNE_WHT, ET_TUNE_MX_F32, ET_TUNE_MX_F64, ET_TUNE_COARSE_LIB, ET_TUNE_ROW_STENCIL_NODES
+_KNOBS = (7, 3, 4, 9, 12)
 
 
 @pytest.fixture
