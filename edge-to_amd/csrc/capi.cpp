@@ -9,6 +9,7 @@
 #include "coarse.hpp"
 #include "common.hpp"
 #include "galerkin_elem.hpp"
+#include "level1.hpp"
 #include "solver.hpp"
 #include "stiffness.hpp"
 
@@ -86,6 +87,8 @@ int et_tune_set(int param, int value) {
     } else if (param == ET_TUNE_WHT_THREADS_F32 || param == ET_TUNE_WHT_THREADS_F64) {
       ET_REQUIRE(value == 512 || value == 1024, "et_tune_set: workgroup size must be 512 or 1024");
       et::set_wht_threads(param == ET_TUNE_WHT_THREADS_F64, value);
+    } else if (param == ET_TUNE_LEVEL1_MF) {
+      et::set_level1_mf(value);
     } else if (param == ET_TUNE_COARSE_LIB) {
       et::set_coarse_lib(value);
     } else if (param == ET_TUNE_WHT) {
@@ -109,6 +112,7 @@ int et_tune_get(int param) {
     return et::get_mx_wgs(param == ET_TUNE_MX_WGS_F64);
   if (param == ET_TUNE_WHT) return et::get_wht();
   if (param == ET_TUNE_COARSE_LIB) return et::get_coarse_lib();
+  if (param == ET_TUNE_LEVEL1_MF) return et::get_level1_mf();
   if (param == ET_TUNE_ROW_STENCIL_NODES) return (int)et::get_row_stencil_nodes();
   if (param == ET_TUNE_WHT_THREADS_F32 || param == ET_TUNE_WHT_THREADS_F64)
     return et::get_wht_threads(param == ET_TUNE_WHT_THREADS_F64);
@@ -413,6 +417,14 @@ int et_mg_profile(et_mg* handle, int enable) {
     handle->impl.prof.reset();
     handle->impl.prof.on = enable != 0;
     return ET_OK;
+  })
+}
+
+int et_mg_apply_level(et_mg* handle, int level, const void* x, void* y) {
+  ET_GUARD({
+    ET_REQUIRE(handle && x && y, "et_mg_apply_level: null pointer");
+    ET_REQUIRE(x != y, "et_mg_apply_level: x and y must not alias");
+    return handle->impl.apply_level(level, x, y);
   })
 }
 

@@ -382,6 +382,11 @@ static int op_launch(const MGLevel<T>& L, const MGShared& sh, int mode, const T*
                      const double* omega, double wscale, const T* xadd, T* y, hipStream_t s) {
   const Grid& g = L.g;
   sh.prof->begin(op_kind(L.level, mode), s);
+  if (L.level == 1 && sh.l1 != nullptr) {
+    const int st = sh.l1->apply<T>(mode, x, b, L.Dinv, omega, wscale, xadd, y, s);
+    sh.prof->end(s);
+    return st;
+  }
   if (L.level == 0 && sizeof(T) == 4 && sh.mfma) {
     const int st = op0_mfma(mode, g, (int64_t)g.ey * g.ez, g.ez, sh.kscale, sh.K0f, sh.Ef, sh.cons,
                             (const float*)x, (const float*)b, (const float*)L.Dinv, omega, wscale,
@@ -487,7 +492,7 @@ int MGHierarchy<T>::vcycle(int l, MGShared& sh, hipStream_t s) {
 }
 
 template <typename T>
-int MGHierarchy<T>::alloc(const Grid& g0, int dof_, int n_level_, hipStream_t s) {
+int MGHierarchy<T>::alloc(const Grid& g0, int dof_, int n_level_, hipStream_t s, bool skip_s1) {
   dof = dof_;
   n_level = n_level_;
   lv.resize(n_level + 1);
@@ -499,7 +504,7 @@ int MGHierarchy<T>::alloc(const Grid& g0, int dof_, int n_level_, hipStream_t s)
     L.n = g.n_nodes() * dof;
     const size_t vb = (size_t)L.n * sizeof(T);
     for (T** p : {&L.b, &L.x, &L.r, &L.e, &L.t, &L.Dinv}) ET_CHECK_HIP(hipMalloc(p, vb));
-    if (l > 0 && l < n_level) {
+    if (l > 0 && l < n_level && !(l == 1 && skip_s1)) {
       const int NH = g.dim == 3 ? 14 : 5;  // centre + forward offsets
       ET_CHECK_HIP(hipMalloc(&L.S, (size_t)NH * dof * dof * g.n_nodes() * sizeof(T)));
     }
@@ -558,6 +563,19 @@ template int level0_op<float>(int, const Grid&, const float*, const float*, cons
                               const float*, float*, hipStream_t);
 
 // ------------------------------------------------------------ et_mg ----
+
+int MGSolver::apply_level(int level, const void* x, void* y) {
+  ET_REQUIRE(built, "et_mg_apply_level: no hierarchy yet (call et_mg_solve first)");
+  ET_REQUIRE(level >= 0 && level < prm.n_level, "et_mg_apply_level: level must be in [0, n_level)");
+  if (prm.precision == 1)
+    ET_TRY(op_launch<float>(h32.lv[level], sh, OP_Y, (const float*)x, nullptr, nullptr, 1.0, nullptr,
+                            (float*)y, stream));
+  else
+    ET_TRY(op_launch<double>(h64.lv[level], sh, OP_Y, (const double*)x, nullptr, nullptr, 1.0, nullptr,
+                             (double*)y, stream));
+  ET_CHECK_HIP(hipStreamSynchronize(stream));
+  return ET_OK;
+}
 
 int MGSolver::profile_read(double* ms, int64_t* launches) {
   ET_CHECK_HIP(hipStreamSynchronize(stream));
@@ -634,10 +652,13 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
       g = g.coarse();
     }
   }
+  use_l1 = elem_galerkin && get_level1_mf() && level1_supported(g0, p.n_level);
+  if (use_l1) ET_TRY(l1.init(g0, K0_host, p.precision == 1, s));
+  sh.l1 = use_l1 ? &l1 : nullptr;
   if (p.precision == 0)
-    ET_TRY(h64.alloc(g0, dof, p.n_level, s));
+    ET_TRY(h64.alloc(g0, dof, p.n_level, s, use_l1));
   else
-    ET_TRY(h32.alloc(g0, dof, p.n_level, s));
+    ET_TRY(h32.alloc(g0, dof, p.n_level, s, use_l1));
   ET_TRY(coarse.init((int)(coarse_grid.n_nodes() * dof), s));
   h64.coarse = &coarse;
   h32.coarse = &coarse;
@@ -655,6 +676,7 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
 
 void MGSolver::destroy() {
   prof.release();
+  l1.release();
   h64.release();
   h32.release();
   coarse.release();
@@ -685,9 +707,10 @@ int MGSolver::setup_levels(MGHierarchy<T>& h) {
   if (!elem_galerkin) ET_TRY(stencil_from_operator(g0, sh.K0d, sh.Ed, sh.cons, S64[0], s));
   Grid g = g0;
   for (int l = 1; l <= L; ++l) {
-    if (l == 1 && elem_galerkin)
+    if (l == 1 && elem_galerkin) {
       ET_TRY(galerkin_from_elements(g0, sh.K0d, Mt, sh.Ed, sh.cons, eflag, S64[1], s));
-    else
+      if (use_l1) ET_TRY(l1.setup(sh.Ed, Mt, eflag, S64[1], s));
+    } else
       ET_TRY(galerkin(g, dof, S64[l - 1], S64[l], s));
     g = g.coarse();
     MGLevel<T>& lev = h.lv[l];
@@ -697,7 +720,7 @@ int MGSolver::setup_levels(MGHierarchy<T>& h) {
     else
       stencil_dinv_kernel<T, 2><<<grid_for(N), kBlock, 0, s>>>(N, S64[l], lev.Dinv);
     ET_CHECK_LAUNCH();
-    if (l < L) {
+    if (l < L && !(l == 1 && use_l1)) {
       // the centre and forward offsets are the trailing blocks of the full stencil
       const int OC = g.dim == 3 ? 13 : 4;
       const int64_t cnt = (int64_t)(OC + 1) * dof * dof * N;
@@ -781,6 +804,7 @@ int MGSolver::solve(const double* E, const uint8_t* cons, const double* b, doubl
   }
   prof.end(s);
   ET_TRY(coarse.check_info(s));
+  built = true;
   ET_CHECK_HIP(hipEventRecord(ev1, s));
   // omega <- min_omega at the start of every solve (GPU semantics)
   pcg_init_kernel<<<1, 1, 0, s>>>(sh.scal, min_omega);

@@ -4,6 +4,7 @@
 
 #include "coarse.hpp"
 #include "common.hpp"
+#include "level1.hpp"
 
 namespace et {
 
@@ -67,6 +68,7 @@ struct MGShared {
   float* Ef = nullptr;         // FP32 copy (mixed precision)
   const uint8_t* cons = nullptr;
   double* scal = nullptr;      // device scalars (SC_*)
+  const Level1MF* l1 = nullptr;  // matrix-free first coarse level, when enabled
   template <typename T>
   const T* K0() const;
   template <typename T>
@@ -96,7 +98,8 @@ struct MGHierarchy {
   std::vector<MGLevel<T>> lv;
   int dof = 0, n_level = 0, n_smooth = 0, cycle_w = 0, w_level_mask = 0;
   CoarseSolver* coarse = nullptr;
-  int alloc(const Grid& g0, int dof, int n_level, hipStream_t s);
+  // skip_s1: level 1 is applied matrix-free (no stencil copy needed)
+  int alloc(const Grid& g0, int dof, int n_level, hipStream_t s, bool skip_s1 = false);
   void release();
   int vcycle(int l, MGShared& sh, hipStream_t s);
 };
@@ -115,6 +118,8 @@ struct MGSolver {
   bool elem_galerkin = false;  // level 1 built from elements (level-0 stencil not formed)
   double* Mt = nullptr;        // child tables W_t^T K0 W_t
   uint8_t* eflag = nullptr;    // per-element "touches a constrained DOF"
+  Level1MF l1;                 // matrix-free level 1 (3-D, even counts, n_level >= 2)
+  bool use_l1 = false;
 
   double *r = nullptr, *z = nullptr, *p_ = nullptr, *q = nullptr, *diag64 = nullptr;
   double* partial = nullptr;
@@ -126,6 +131,8 @@ struct MGSolver {
   double t_setup_ms = 0, t_pcg_ms = 0;
   Prof prof;
   int profile_read(double* ms, int64_t* launches);
+  int apply_level(int level, const void* x, void* y);
+  bool built = false;  // a solve has set up the hierarchy
 
   int create(const et_mg_params& p, const double* K0_host, hipStream_t s);
   void destroy();

@@ -64,7 +64,10 @@ const char* et_version(void);
  * ET_TUNE_WHT_THREADS_F32 / _F64: workgroup size (512 or 1024) of the transformed-basis
  *   operator.
  * ET_TUNE_ROW_STENCIL_NODES: coarse levels with at most this many nodes apply their
- *   stencil with one thread per (node, component) instead of one per node. */
+ *   stencil with one thread per (node, component) instead of one per node.
+ * ET_TUNE_LEVEL1_MF: 1 (default) = the first coarse level of 3-D hierarchies with even
+ *   element counts is applied matrix-free from the element densities; 0 = stored stencil.
+ *   Read by et_mg_create. */
 enum {
   ET_TUNE_TILED_JC = 0,
   ET_TUNE_MFMA_JC = 1,
@@ -78,7 +81,8 @@ enum {
   ET_TUNE_COARSE_LIB = 9,
   ET_TUNE_WHT_THREADS_F32 = 10,
   ET_TUNE_WHT_THREADS_F64 = 11,
-  ET_TUNE_ROW_STENCIL_NODES = 12
+  ET_TUNE_ROW_STENCIL_NODES = 12,
+  ET_TUNE_LEVEL1_MF = 13
 };
 int et_tune_set(int param, int value);
 int et_tune_get(int param);
@@ -245,6 +249,12 @@ int et_mg_destroy(et_mg* handle);
  * reference does.  Synchronises `stream`.  Outputs: iterations, |r|/|b|. */
 int et_mg_solve(et_mg* handle, const double* E, const uint8_t* cons, const double* b,
                 double* x, double tol, int maxiter, int* iters, double* residual);
+
+/* y = A_l x with the operator of multigrid level l (0 = fine grid, up to n_level - 1)
+ * as the last et_mg_solve built it, in the V-cycle's precision (double for precision
+ * 0, float for 1); x and y are device vectors of that level.  No counterpart in the
+ * reference: exposes each level's operator for verification.  Synchronises. */
+int et_mg_apply_level(et_mg* handle, int level, const void* x, void* y);
 
 /* Per-phase timings of the last solve in milliseconds (setup, pcg), via HIP events. */
 int et_mg_last_timings(et_mg* handle, double* setup_ms, double* pcg_ms);

@@ -39,9 +39,11 @@ def _oracle_problem(nel, nu, rmin, clamp, load, fy):
 
 
 # fine-grid operator variants: (ET_TUNE_WHT, ET_TUNE_MX_F32, ET_TUNE_MX_F64); None = library default
-_OPS = {"default": None, "tiled": (0, 0, 0, 0, 0), "matrix": (0, 1, 1, 1, 1 << 30), "wht": (1, 0, 0, 1, 300000)}
-# ET_TUNE_WHT, ET_TUNE_MX_F32, ET_TUNE_MX_F64, ET_TUNE_COARSE_LIB, ET_TUNE_ROW_STENCIL_NODES
-_KNOBS = (7, 3, 4, 9, 12)
+_OPS = {"default": None, "tiled": (0, 0, 0, 0, 0, 0), "matrix": (0, 1, 1, 1, 1 << 30, 1),
+        "wht": (1, 0, 0, 1, 300000, 1)}
+# ET_TUNE_WHT, ET_TUNE_MX_F32, ET_TUNE_MX_F64, ET_TUNE_COARSE_LIB, ET_TUNE_ROW_STENCIL_NODES,
+# ET_TUNE_LEVEL1_MF
+_KNOBS = (7, 3, 4, 9, 12, 13)
 
 
 @pytest.fixture
@@ -116,3 +118,48 @@ def test_simp_trajectory_2d():
     comp_ref = np.array(ohist["comp"])
     assert np.max(np.abs(comp_gpu - comp_ref) / comp_ref) < 1e-6
     assert np.max(np.abs(x.cpu().numpy() - xo)) < 1e-4
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("level1_mf", [1, 0])
+@pytest.mark.parametrize("nel", [(16, 8, 8), (20, 12, 10)])
+def test_level_operators_match_galerkin(precision, level1_mf, nel):
+    """Every smoothing level's operator (fine grid, matrix-free or stencil first coarse level,
+    stencil levels) against the oracle's Galerkin products R A R^T on a clamped cantilever with
+    random densities, so the constrained-DOF correction of the first coarse level is exercised."""
+    ET = _edgeto()
+    from edgeto_amd import _lib as L
+    old = L.lib().et_tune_get(13)
+    L.call("et_tune_set", 13, level1_mf)
+    try:
+        mesh, kernel, filt, clamp, load = _cantilever_3d(ET, nel)
+        rng = np.random.default_rng(3)
+        E = rng.uniform(0.01, 1.0, mesh.n_el)
+        kernel.set_rho(torch.tensor(E, device="cuda"))
+        tp = ET.CuTopOpt(mesh, ET.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=0.3), kernel,
+                         None, filt, verbose=False)
+        tp.add_BC_nodal(clamp, np.array([[1, 1, 1]]))
+        tp.add_F_nodal(load, np.array([[0.0, -1.0 / len(load), 0.0]]))
+        n_level = 3
+        solver = ET.CuMultiGrid(mesh, kernel, tol=1e-6, n_smooth=2, n_level=n_level, precision=precision)
+        solver.solve(tp.f, use_last=False)
+    finally:
+        L.call("et_tune_set", 13, old)
+    prob = _oracle_problem(nel, 0.3, 2.0, clamp, load, -1.0 / len(load))
+    A = oops.assemble(prob.K0, prob.elements, E, prob.cons, prob.n_dof).tocsr()
+    from oracle import mgm as omgm
+    dt = torch.float32 if precision == "mixed" else torch.float64
+    tol = 2e-5 if precision == "mixed" else 1e-11
+    lnel = tuple(nel)
+    for level in range(n_level):
+        if level > 0:
+            R = omgm.restriction_matrix(lnel, 3)
+            A = (R @ A @ R.T).tocsr()
+            lnel = tuple(n // 2 for n in lnel)
+        x = rng.standard_normal(A.shape[0])
+        xt = torch.tensor(x, dtype=dt, device="cuda")
+        yt = torch.empty_like(xt)
+        L.call("et_mg_apply_level", solver._handle, level, L.ptr(xt), L.ptr(yt))
+        ref = A @ x
+        err = np.abs(yt.cpu().numpy() - ref).max() / np.abs(ref).max()
+        assert err < tol, (level, err)
