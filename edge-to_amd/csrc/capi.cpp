@@ -415,7 +415,7 @@ int et_mg_profile(et_mg* handle, int enable) {
     ET_REQUIRE(handle, "et_mg_profile: null handle");
     ET_CHECK_HIP(hipStreamSynchronize(handle->impl.stream));
     handle->impl.prof.reset();
-    handle->impl.prof.on = enable != 0;
+    handle->impl.prof.mask = (unsigned)enable & ((1u << ET_PROF_KINDS) - 1u);
     return ET_OK;
   })
 }

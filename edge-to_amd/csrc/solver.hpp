@@ -13,7 +13,7 @@ enum { SC_BB = 0, SC_RHO, SC_RHO_NEW, SC_PQ, SC_RR, SC_OMEGA, SC_R, SC_STATUS, S
 
 // Event-based per-kernel-class timing (enabled on demand).
 struct Prof {
-  bool on = false;
+  unsigned mask = 0;  // bit k: time kernel class k
   struct Rec {
     int kind;
     hipEvent_t a, b;
@@ -31,7 +31,7 @@ struct Prof {
     return pool[next++];
   }
   void begin(int kind, hipStream_t s) {
-    if (!on || recs.size() > 200000) return;
+    if (!((mask >> kind) & 1u) || recs.size() > 200000) return;
     Rec r{kind, ev(), ev()};
     if (!r.a || !r.b) return;
     (void)hipEventRecord(r.a, s);
@@ -39,7 +39,7 @@ struct Prof {
     open_kind = kind;
   }
   void end(hipStream_t s) {
-    if (!on || open_kind < 0) return;
+    if (open_kind < 0) return;
     (void)hipEventRecord(recs.back().b, s);
     open_kind = -1;
   }

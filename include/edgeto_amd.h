@@ -260,7 +260,10 @@ int et_mg_apply_level(et_mg* handle, int level, const void* x, void* y);
 int et_mg_last_timings(et_mg* handle, double* setup_ms, double* pcg_ms);
 
 /* Kernel-class profiling with HIP events recorded around each launch on the
- * solver's stream.  et_mg_profile(h, 1) clears and enables, (h, 0) disables.
+ * solver's stream.  et_mg_profile(h, mask) clears the records and times the
+ * classes whose bit (1 << ET_PROF_*) is set in mask: -1 times every class, 0
+ * disables.  Each timed launch costs two event records on the stream, so time
+ * only the classes needed inside a measured region.
  * et_mg_profile_read synchronises and fills ms[k] / launches[k] for the
  * ET_PROF_KINDS classes below (accumulated since the last enable). */
 enum {
