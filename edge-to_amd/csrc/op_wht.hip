@@ -48,6 +48,7 @@ namespace et {
 // values, (TC-1)*nz <= 1024 output nodes.
 constexpr int kWStageCap = 1536, kWECap = 1024, kWNodeCap = 1024;
 constexpr int kWG = 63;  // transformed blocks M = 1..7, 3 x 3 each
+__device__ const uint8_t kWZeroByte[4] = {0, 0, 0, 0};
 
 // workgroup of NT threads (512 or 1024)
 template <int NT>
@@ -136,7 +137,12 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
   T* se = ss + 12 * tc;                   // [tc][nelz]
   T* sg = se + tc * nelz;                 // G, 63 entries
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t plane = (int64_t)nx * nz;
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+  // 32-bit DOF offsets: the host checks 3 * n_nodes < 2^31
+  const int plane3 = nx * nz * 3;
+  // without constraints every constraint-byte read goes to a zero byte
+  const uint8_t* cp = cons != nullptr ? cons : kWZeroByte;
+  const int cmask = cons != nullptr ? ~0 : 0;
   constexpr int kWThreads = NT;
   constexpr int kWSlots = WCfg<NT>::slots;
   constexpr int kWStage = WCfg<NT>::stage;
@@ -213,25 +219,29 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
     un += jb - ja;
     const int c0 = xt * (tc - 1) - 1;  // element column of local column 0 = node column of staged column 0
 
+    // Staging loads are unconditional (out-of-range slots read index 0) and their
+    // masks are applied when the registers are written to LDS, so a layer's loads
+    // are all in flight together instead of each waiting on its constraint byte.
     T pu[kWStage][3];
+    uint8_t pk[kWStage][3];
+    bool pv[kWStage];
     T pe[kWEStage];
+    bool pev[kWEStage];
     auto fetch_u = [&](int jp) {
       const bool pin = jp >= 0 && jp < ny;
-      const int64_t pb = (int64_t)(pin ? jp : 0) * plane * 3 + (int64_t)c0 * nz * 3;
+      const int pb = (pin ? jp : 0) * plane3 + c0 * nz * 3;
 #pragma unroll
       for (int m = 0; m < kWStage; ++m) {
         const int s = tid + kWThreads * m;
         const int n = c0 + snl[m];
-        const bool in = pin && s < span && n >= 0 && n < nx;
+        pv[m] = pin && s < span && n >= 0 && n < nx;
+        if (m == 0 || kWThreads * m + wave_u * 64 < span) {  // wave-uniform
+          const int d = pv[m] ? pb + s * 3 : 0;
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          T v = T(0);
-          if (in) {
-            const int64_t d = pb + (int64_t)s * 3 + r;
-            v = x[d];
-            if (cons != nullptr && cons[d]) v = T(0);
+          for (int r = 0; r < 3; ++r) {
+            pu[m][r] = x[d + r];
+            pk[m][r] = cp[(d + r) & cmask];
           }
-          pu[m][r] = v;
         }
       }
     };
@@ -242,7 +252,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
         const int s = tid + kWThreads * m;
         if (s < span) {
 #pragma unroll
-          for (int r = 0; r < 3; ++r) dst[r * span + s] = pu[m][r];
+          for (int r = 0; r < 3; ++r) dst[r * span + s] = (pv[m] && pk[m][r] == 0) ? pu[m][r] : T(0);
         }
       }
     };
@@ -251,15 +261,15 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
 #pragma unroll
       for (int m = 0; m < kWEStage; ++m) {
         const int c = c0 + ecl[m];
-        const bool in = lin && ecl[m] < tc && c >= 0 && c < nelx;
-        pe[m] = in ? E[(int64_t)c * A.e_sx + (int64_t)ey * A.e_sy + eez[m]] : T(0);
+        pev[m] = lin && ecl[m] < tc && c >= 0 && c < nelx;
+        pe[m] = E[pev[m] ? c * (int)A.e_sx + ey * (int)A.e_sy + eez[m] : 0];
       }
     };
     auto store_e = [&]() {
 #pragma unroll
       for (int m = 0; m < kWEStage; ++m) {
         const int t = tid + kWThreads * m;
-        if (t < tc * nelz) se[t] = pe[m];
+        if (t < tc * nelz) se[t] = pev[m] ? pe[m] : T(0);
       }
     };
     // face transform of the face of plane jp (slot jp & 1) for element slot m
@@ -304,22 +314,18 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
         fetch_e(ey + 1);
       }
       // global operands of this layer's output plane, requested before the element phase
-      const int64_t jb3 = (int64_t)ey * plane * 3 + (int64_t)c0 * nz * 3;
+      const int jb3 = ey * plane3 + c0 * nz * 3;
       bool act[kWNodes];
-      int64_t dd[kWNodes];
+      int dd[kWNodes];
       T bq[kWNodes][3], dq[kWNodes][3], aq[kWNodes][3], xq[kWNodes][3];
       uint8_t cq[kWNodes][3];
-      // the node's own (staged, constrained-zeroed) value is read here, before the
-      // barrier: after it, the slot of plane ey is refilled by other threads
-      const T* own = su + (size_t)(ey & 1) * 3 * span;
 #pragma unroll
       for (int q = 0; q < kWNodes; ++q) {
         act[q] = out && onl[q] < tc && c0 + onl[q] < nx;
-        dd[q] = act[q] ? jb3 + (int64_t)osl[q] * 3 : 0;
+        dd[q] = act[q] ? jb3 + osl[q] * 3 : 0;
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-          xq[q][r] = (MODE != TM_Y && act[q]) ? own[r * span + osl[q]] : T(0);
-          cq[q][r] = cons != nullptr ? cons[dd[q] + r] : (uint8_t)0;
+          cq[q][r] = cp[(dd[q] + r) & cmask];
           bq[q][r] = MODE != TM_Y ? b[dd[q] + r] : T(0);
           dq[q][r] = MODE >= TM_JAC ? Dinv[dd[q] + r] : T(0);
           aq[q][r] = MODE == TM_JAC_ADD ? xadd[dd[q] + r] : T(0);
@@ -388,6 +394,17 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
             if (out && lane == 63) ss[((c * 2 + hx) * tc + cl) * 2 + (ez >> 6)] = up;
           }
       }
+      // the node's own (staged, constrained-zeroed) value is read before the
+      // barrier: after it, the slot of plane ey is refilled by other threads
+      if (MODE != TM_Y) {
+        const T* own = su + (size_t)(ey & 1) * 3 * span;
+#pragma unroll
+        for (int q = 0; q < kWNodes; ++q) {
+          const int os = min(osl[q], span - 1);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) xq[q][r] = own[r * span + os];
+        }
+      }
       __syncthreads();
 
       // node phase: node plane ey
@@ -400,7 +417,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
             const bool seam = k > 0 && (k & 63) == 0;
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-              const int64_t d = dd[q] + r;
+              const int d = dd[q] + r;
               T ax0 = T(0);
               if (wv) ax0 = sw[((r * 2) * tc + nl) * nz + k] + sw[((r * 2 + 1) * tc + nl - 1) * nz + k];
               if (seam)
@@ -446,7 +463,7 @@ int get_wht_wgs() { return g_wht_wgs; }
 constexpr size_t kWMaxLds = 160 * 1024;
 
 bool wht_supported(const Grid& g, size_t elem_bytes) {
-  if (g.dim != 3 || g.ez > 128) return false;
+  if (g.dim != 3 || g.ez > 128 || 3 * g.n_nodes() >= ((int64_t)1 << 31)) return false;
   const int tc = wht_tc(g);
   return (tc + 1) * g.nz() <= kWStageCap && tc * g.ez <= kWECap && (tc - 1) * g.nz() <= kWNodeCap &&
          wht_lds_values(g) * elem_bytes <= kWMaxLds;
