@@ -217,13 +217,17 @@ __global__ __launch_bounds__(256) void l1_node_kernel(const Grid g1, const T* __
   const int k = n % nz, ij = n / nz, i = ij % nx, j = ij / nx;
   T acc[3] = {T(0), T(0), T(0)};
   const int64_t ne = g1.n_elem();
+  // branch-free: missing elements read element 0 with weight zero
 #pragma unroll
   for (int beta = 0; beta < 8; ++beta) {
     const int I = i - (beta & 1), J = j - ((beta >> 1) & 1), K = k - (beta >> 2);
-    if (I < 0 || I >= g1.ex || J < 0 || J >= g1.ey || K < 0 || K >= g1.ez) continue;
-    const int64_t C = (int64_t)(I * g1.ey + J) * g1.ez + K;
+    const bool in = I >= 0 && I < g1.ex && J >= 0 && J < g1.ey && K >= 0 && K < g1.ez;
+    const int64_t C = in ? (int64_t)(I * g1.ey + J) * g1.ez + K : 0;
 #pragma unroll
-    for (int r = 0; r < 3; ++r) acc[r] += part[(beta * 3 + r) * ne + C];
+    for (int r = 0; r < 3; ++r) {
+      const T v = part[(beta * 3 + r) * ne + C];
+      acc[r] += in ? v : T(0);
+    }
   }
   const T s = (T)l1_inv_tw(g1, i, j, k);
 #pragma unroll

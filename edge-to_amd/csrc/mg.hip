@@ -53,6 +53,8 @@ __device__ __forceinline__ void node_coords32(int n, int nx, int nz, int& i, int
   j = n / nx;
 }
 
+// Branch-free: out-of-range neighbours read a clamped (in-range) node with
+// weight 0, so all loads of a thread are in flight together.
 template <typename T, int DIM, int DOF>
 __global__ __launch_bounds__(256) void restrict_kernel(const Grid gf, const Grid gc,
                                                        const T* __restrict__ vf,
@@ -67,26 +69,29 @@ __global__ __launch_bounds__(256) void restrict_kernel(const Grid gf, const Grid
   for (int d = 0; d < DOF; ++d) acc[d] = T(0);
 #pragma unroll
   for (int dj = -1; dj <= 1; ++dj) {
-    const int fj = 2 * cj + dj;
-    if (fj < 0 || fj >= fny) continue;
+    const int fj0 = 2 * cj + dj;
+    const bool jin = fj0 >= 0 && fj0 < fny;
+    const int fj = min(max(fj0, 0), fny - 1);
 #pragma unroll
     for (int di = -1; di <= 1; ++di) {
-      const int fi = 2 * ci + di;
-      if (fi < 0 || fi >= fnx) continue;
+      const int fi0 = 2 * ci + di;
+      const bool iin = fi0 >= 0 && fi0 < fnx;
+      const int fi = min(max(fi0, 0), fnx - 1);
 #pragma unroll
       for (int dk = (DIM == 3 ? -1 : 0); dk <= (DIM == 3 ? 1 : 0); ++dk) {
-        const int fk = 2 * ck + dk;
-        if (DIM == 3 && (fk < 0 || fk >= fnz)) continue;
-        const T wgt = (T)(pw(di) * pw(dj) * pw(dk));
+        const int fk0 = 2 * ck + dk;
+        const bool kin = DIM != 3 || (fk0 >= 0 && fk0 < fnz);
+        const int fk = DIM == 3 ? min(max(fk0, 0), fnz - 1) : 0;
+        const T wgt = (jin && iin && kin) ? (T)(pw(di) * pw(dj) * pw(dk)) : T(0);
         const int f = DIM == 3 ? (fj * fnx + fi) * fnz + fk : fj * fnx + fi;
 #pragma unroll
-        for (int d = 0; d < DOF; ++d) acc[d] += wgt * vf[(int64_t)f * DOF + d];
+        for (int d = 0; d < DOF; ++d) acc[d] += wgt * vf[f * DOF + d];
       }
     }
   }
   const T inv = (T)(1.0 / tw_of<DIM>(gc, ci, cj, ck));
 #pragma unroll
-  for (int d = 0; d < DOF; ++d) vc[(int64_t)c * DOF + d] = acc[d] * inv;
+  for (int d = 0; d < DOF; ++d) vc[c * DOF + d] = acc[d] * inv;
 }
 
 template <typename T>
@@ -111,6 +116,9 @@ template int restrict_op<float>(const Grid&, int, const float*, float*, hipStrea
 // One thread per fine node: average of the (1, 2, 4 or 8) in-range coarse
 // nodes of its cell, each divided by its total weight tw.
 
+// Branch-free: the cell's up to 8 coarse corners are read at clamped indices;
+// an axis contributes its second corner (weight 1) only when the fine node is
+// odd on it and that corner exists.
 template <typename T, int DIM, int DOF>
 __global__ __launch_bounds__(256) void prolong_kernel(const Grid gf, const Grid gc,
                                                       const T* __restrict__ vc,
@@ -121,27 +129,30 @@ __global__ __launch_bounds__(256) void prolong_kernel(const Grid gf, const Grid 
   int i, j, k;
   node_coords32<DIM>(f, gf.nx(), gf.nz(), i, j, k);
   const int ci0 = i >> 1, cj0 = j >> 1, ck0 = k >> 1;
-  const int ni = (i & 1) && ci0 + 1 < cnx ? 2 : 1;
-  const int nj = (j & 1) && cj0 + 1 < cny ? 2 : 1;
-  const int nk = DIM == 3 && (k & 1) && ck0 + 1 < cnz ? 2 : 1;
+  const bool two_i = (i & 1) && ci0 + 1 < cnx;
+  const bool two_j = (j & 1) && cj0 + 1 < cny;
+  const bool two_k = DIM == 3 && (k & 1) && ck0 + 1 < cnz;
+  const int ci1 = two_i ? ci0 + 1 : ci0, cj1 = two_j ? cj0 + 1 : cj0, ck1 = two_k ? ck0 + 1 : ck0;
   T acc[DOF];
 #pragma unroll
   for (int d = 0; d < DOF; ++d) acc[d] = T(0);
-  int count = 0;
-  for (int a = 0; a < nj; ++a)
-    for (int bq = 0; bq < ni; ++bq)
-      for (int cq = 0; cq < nk; ++cq) {
-        const int ci = ci0 + bq, cj = cj0 + a, ck = ck0 + cq;
-        if (ci >= cnx || cj >= cny || (DIM == 3 && ck >= cnz)) continue;
-        const T inv = (T)(1.0 / tw_of<DIM>(gc, ci, cj, ck));
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int bq = 0; bq < 2; ++bq)
+#pragma unroll
+      for (int cq = 0; cq < (DIM == 3 ? 2 : 1); ++cq) {
+        const bool use = (a == 0 || two_j) && (bq == 0 || two_i) && (cq == 0 || two_k);
+        const int ci = bq ? ci1 : ci0, cj = a ? cj1 : cj0, ck = cq ? ck1 : ck0;
+        const T inv = use ? (T)(1.0 / tw_of<DIM>(gc, ci, cj, ck)) : T(0);
         const int c = DIM == 3 ? (cj * cnx + ci) * cnz + ck : cj * cnx + ci;
 #pragma unroll
-        for (int d = 0; d < DOF; ++d) acc[d] += vc[(int64_t)c * DOF + d] * inv;
-        ++count;
+        for (int d = 0; d < DOF; ++d) acc[d] += vc[c * DOF + d] * inv;
       }
-  const T sc = count > 0 ? T(1) / (T)count : T(0);
+  const int count = (two_i ? 2 : 1) * (two_j ? 2 : 1) * (two_k ? 2 : 1);
+  const T sc = T(1) / (T)count;
 #pragma unroll
-  for (int d = 0; d < DOF; ++d) vf[(int64_t)f * DOF + d] = acc[d] * sc;
+  for (int d = 0; d < DOF; ++d) vf[f * DOF + d] = acc[d] * sc;
 }
 
 template <typename T>

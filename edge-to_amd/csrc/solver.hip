@@ -128,39 +128,38 @@ __global__ __launch_bounds__(256) void opS_kernel(const Grid g, const T* __restr
   T acc[DIM];
 #pragma unroll
   for (int r = 0; r < DIM; ++r) acc[r] = T(0);
-#pragma unroll
+  // two offsets per step: ~50 independent loads in flight without spilling
+#pragma unroll 2
   for (int h = 0; h < NH; ++h) {
     const int o = OC + h;
     const int oi = DIM == 3 ? o / 9 - 1 : o / 3 - 1;
     const int oj = DIM == 3 ? (o / 3) % 3 - 1 : o % 3 - 1;
     const int ok = DIM == 3 ? o % 3 - 1 : 0;
     const int64_t off = DIM == 3 ? ((int64_t)oj * nx + oi) * nz + ok : (int64_t)oj * nx + oi;
+    // Branch-free: an out-of-grid neighbour reads node n itself with its x
+    // weighted by zero, so all loads of the thread can be in flight at once.
     // forward block S_o(n) x_{n+o}
     {
       const int mi = i + oi, mj = j + oj, mk = k + ok;
       const bool in = mi >= 0 && mi < nx && mj >= 0 && mj < ny && (DIM == 2 || (mk >= 0 && mk < nz));
-      if (in) {
-        const int64_t m = n + off;
+      const int64_t m = in ? n + off : n;
 #pragma unroll
-        for (int q = 0; q < DIM; ++q) {
-          const T xv = x[m * DIM + q];
+      for (int q = 0; q < DIM; ++q) {
+        const T xv = in ? x[m * DIM + q] : T(0);
 #pragma unroll
-          for (int r = 0; r < DIM; ++r) acc[r] += S[((int64_t)(h * DIM + r) * DIM + q) * N + n] * xv;
-        }
+        for (int r = 0; r < DIM; ++r) acc[r] += S[((int64_t)(h * DIM + r) * DIM + q) * N + n] * xv;
       }
     }
     // backward block S_o(n-o)^T x_{n-o}
     if (h > 0) {
       const int mi = i - oi, mj = j - oj, mk = k - ok;
       const bool in = mi >= 0 && mi < nx && mj >= 0 && mj < ny && (DIM == 2 || (mk >= 0 && mk < nz));
-      if (in) {
-        const int64_t m = n - off;
+      const int64_t m = in ? n - off : n;
 #pragma unroll
-        for (int q = 0; q < DIM; ++q) {
-          const T xv = x[m * DIM + q];
+      for (int q = 0; q < DIM; ++q) {
+        const T xv = in ? x[m * DIM + q] : T(0);
 #pragma unroll
-          for (int r = 0; r < DIM; ++r) acc[r] += S[((int64_t)(h * DIM + q) * DIM + r) * N + m] * xv;
-        }
+        for (int r = 0; r < DIM; ++r) acc[r] += S[((int64_t)(h * DIM + q) * DIM + r) * N + m] * xv;
       }
     }
   }
@@ -196,22 +195,25 @@ __global__ __launch_bounds__(256) void opS_row_kernel(const Grid g, const T* __r
     const int oj = DIM == 3 ? (o / 3) % 3 - 1 : o % 3 - 1;
     const int ok = DIM == 3 ? o % 3 - 1 : 0;
     const int off = DIM == 3 ? (oj * nx + oi) * nz + ok : oj * nx + oi;
+    // branch-free as in opS_kernel
     {
       const int mi = i + oi, mj = j + oj, mk = k + ok;
-      if (mi >= 0 && mi < nx && mj >= 0 && mj < ny && (DIM == 2 || (mk >= 0 && mk < nz))) {
-        const int m = n + off;
+      const bool in = mi >= 0 && mi < nx && mj >= 0 && mj < ny && (DIM == 2 || (mk >= 0 && mk < nz));
+      const int m = in ? n + off : n;
 #pragma unroll
-        for (int q = 0; q < DIM; ++q)
-          acc += S[((int64_t)(h * DIM + r) * DIM + q) * N + n] * x[(int64_t)m * DIM + q];
+      for (int q = 0; q < DIM; ++q) {
+        const T xv = in ? x[m * DIM + q] : T(0);
+        acc += S[((h * DIM + r) * DIM + q) * N + n] * xv;
       }
     }
     if (h > 0) {
       const int mi = i - oi, mj = j - oj, mk = k - ok;
-      if (mi >= 0 && mi < nx && mj >= 0 && mj < ny && (DIM == 2 || (mk >= 0 && mk < nz))) {
-        const int m = n - off;
+      const bool in = mi >= 0 && mi < nx && mj >= 0 && mj < ny && (DIM == 2 || (mk >= 0 && mk < nz));
+      const int m = in ? n - off : n;
 #pragma unroll
-        for (int q = 0; q < DIM; ++q)
-          acc += S[((int64_t)(h * DIM + q) * DIM + r) * N + m] * x[(int64_t)m * DIM + q];
+      for (int q = 0; q < DIM; ++q) {
+        const T xv = in ? x[m * DIM + q] : T(0);
+        acc += S[((h * DIM + q) * DIM + r) * N + m] * xv;
       }
     }
   }
@@ -228,7 +230,8 @@ __global__ __launch_bounds__(256) void opS_row_kernel(const Grid g, const T* __r
   y[d] = v;
 }
 
-static int64_t g_row_stencil_max_nodes = 300000;  // coarse levels up to this size use opS_row_kernel
+// coarse levels up to this size use opS_row_kernel (32-bit stencil offsets: also below 2^31 / 126 nodes)
+static int64_t g_row_stencil_max_nodes = 300000;
 void set_row_stencil_nodes(int64_t n) { g_row_stencil_max_nodes = n; }
 int64_t get_row_stencil_nodes() { return g_row_stencil_max_nodes; }
 
@@ -406,7 +409,7 @@ static int op_launch(const MGLevel<T>& L, const MGShared& sh, int mode, const T*
                                                    L.Dinv, omega, wscale, xadd, y)
 #define ET_OPS(DIM, MODE)                                                                            \
   do {                                                                                               \
-    if (g.n_nodes() <= g_row_stencil_max_nodes)                                                      \
+    if (g.n_nodes() <= g_row_stencil_max_nodes && g.n_nodes() * 126 < ((int64_t)1 << 31))          \
       opS_row_kernel<T, DIM, MODE><<<grid_for(g.n_nodes() * DIM), kBlock, 0, s>>>(g, L.S, x, b, L.Dinv, \
                                                                                   omega, wscale, xadd, y); \
     else                                                                                             \
