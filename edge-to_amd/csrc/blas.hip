@@ -32,6 +32,12 @@ int reduce_finalize(const double* partial, double* out, hipStream_t s) {
   return ET_OK;
 }
 
+int reduce_finalize_n(const double* partial, int m, double* out, hipStream_t s) {
+  finalize_kernel<<<1, 256, 0, s>>>(m, partial, out);
+  ET_CHECK_LAUNCH();
+  return ET_OK;
+}
+
 int dot(int64_t n, const double* a, const double* b, double* partial, double* out, hipStream_t s) {
   dot_partial_kernel<<<kRedBlocks, 256, 0, s>>>(n, a, b, partial);
   ET_CHECK_LAUNCH();

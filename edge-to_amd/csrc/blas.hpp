@@ -9,5 +9,7 @@ constexpr int kRedBlocks = 1024;  // fixed first-stage grid of every reduction
 // partial: kRedBlocks doubles of device scratch; out: device scalar
 int dot(int64_t n, const double* a, const double* b, double* partial, double* out, hipStream_t s);
 int reduce_finalize(const double* partial, double* out, hipStream_t s);
+// the same over the first m partials
+int reduce_finalize_n(const double* partial, int m, double* out, hipStream_t s);
 
 }  // namespace et

@@ -54,6 +54,16 @@ __device__ __forceinline__ double l1_inv_tw(const Grid& g, int i, int j, int k) 
   return 1.0 / t;
 }
 
+// the same weight in the working precision, as a product of per-axis factors
+template <typename T>
+__device__ __forceinline__ T l1_inv_tw_t(const Grid& g, int i, int j, int k) {
+  const T bnd = T(2.0 / 3.0), mid = T(0.5);
+  T t = (i == 0 || i == g.nx() - 1) ? bnd : mid;
+  t *= (j == 0 || j == g.ny() - 1) ? bnd : mid;
+  t *= (k == 0 || k == g.nz() - 1) ? bnd : mid;
+  return t;
+}
+
 // in-place 8-point Walsh-Hadamard transform, natural order
 template <typename T>
 __device__ __forceinline__ void l1_wht8(T (&v)[8]) {
@@ -184,7 +194,7 @@ __global__ __launch_bounds__(256) void l1_elem_kernel(const Grid g1, const T* __
   for (int beta = 0; beta < 8; ++beta) {
     const int i = I + (beta & 1), j = J + ((beta >> 1) & 1), k = K + (beta >> 2);
     const int64_t n = ((int64_t)j * nx + i) * nz + k;
-    const T s = (T)l1_inv_tw(g1, i, j, k);
+    const T s = l1_inv_tw_t<T>(g1, i, j, k);
 #pragma unroll
     for (int c = 0; c < 3; ++c) xh[c][beta] = x[n * 3 + c] * s;
   }
@@ -229,7 +239,7 @@ __global__ __launch_bounds__(256) void l1_node_kernel(const Grid g1, const T* __
       acc[r] += in ? v : T(0);
     }
   }
-  const T s = (T)l1_inv_tw(g1, i, j, k);
+  const T s = l1_inv_tw_t<T>(g1, i, j, k);
 #pragma unroll
   for (int r = 0; r < 3; ++r) acc[r] *= s;
   if (aidx[n] >= 0) {

@@ -37,6 +37,17 @@ __device__ __forceinline__ double tw_of(const Grid& gc, int i, int j, int k) {
   return t;
 }
 
+// 1/tw(c) as a product of per-axis factors (2/3 on the boundary, 1/2 inside),
+// in the working precision: no division in the transfer kernels
+template <typename T, int DIM>
+__device__ __forceinline__ T inv_tw_of(const Grid& gc, int i, int j, int k) {
+  const T bnd = T(2.0 / 3.0), mid = T(0.5);
+  T t = (i == 0 || i == gc.nx() - 1) ? bnd : mid;
+  t *= (j == 0 || j == gc.ny() - 1) ? bnd : mid;
+  if (DIM == 3) t *= (k == 0 || k == gc.nz() - 1) ? bnd : mid;
+  return t;
+}
+
 // ------------------------------------------------------------ restriction ---
 // One thread per coarse node, all DOF components at once; 32-bit node
 // arithmetic (node counts are below 2^31 by construction of the Grid).
@@ -89,7 +100,7 @@ __global__ __launch_bounds__(256) void restrict_kernel(const Grid gf, const Grid
       }
     }
   }
-  const T inv = (T)(1.0 / tw_of<DIM>(gc, ci, cj, ck));
+  const T inv = inv_tw_of<T, DIM>(gc, ci, cj, ck);
 #pragma unroll
   for (int d = 0; d < DOF; ++d) vc[c * DOF + d] = acc[d] * inv;
 }
@@ -144,7 +155,7 @@ __global__ __launch_bounds__(256) void prolong_kernel(const Grid gf, const Grid 
       for (int cq = 0; cq < (DIM == 3 ? 2 : 1); ++cq) {
         const bool use = (a == 0 || two_j) && (bq == 0 || two_i) && (cq == 0 || two_k);
         const int ci = bq ? ci1 : ci0, cj = a ? cj1 : cj0, ck = cq ? ck1 : ck0;
-        const T inv = use ? (T)(1.0 / tw_of<DIM>(gc, ci, cj, ck)) : T(0);
+        const T inv = use ? inv_tw_of<T, DIM>(gc, ci, cj, ck) : T(0);
         const int c = DIM == 3 ? (cj * cnx + ci) * cnz + ck : cj * cnx + ci;
 #pragma unroll
         for (int d = 0; d < DOF; ++d) acc[d] += vc[c * DOF + d] * inv;

@@ -37,7 +37,8 @@ constexpr int kESlots = 4;  // E values staged per thread per plane (e_span <= 1
 // offsets, so it is fetched with scalar loads into SGPRs.
 #define ET_TILED_K(r, m, c) K0[(r) * 24 + 3 * (m) + (c)]
 
-enum TiledMode { TM_Y = 0, TM_RES = 1, TM_JAC = 2, TM_JAC_ADD = 3 };
+// TM_YDOT (transformed-basis operator only): y = A x plus per-workgroup partial sums of x.y
+enum TiledMode { TM_Y = 0, TM_RES = 1, TM_JAC = 2, TM_JAC_ADD = 3, TM_YDOT = 4 };
 
 struct TiledArgs {
   Grid g;

@@ -123,8 +123,10 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
                                                             const T* __restrict__ Dinv,
                                                             const double* __restrict__ omega,
                                                             double wscale, const T* __restrict__ xadd,
-                                                            T* __restrict__ y) {
+                                                            T* __restrict__ y, double* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
+  constexpr bool kJac = MODE == TM_JAC || MODE == TM_JAC_ADD;
+  constexpr bool kProd = MODE == TM_Y || MODE == TM_YDOT;  // output is A x
   const Grid& g = A.g;
   const int nx = g.nx(), ny = g.ny(), nz = g.nz();
   const int nelx = g.ex, nely = g.ey, nelz = g.ez;
@@ -175,7 +177,8 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
     if (wht_nz(M, c, d)) gk[t] = wht_uniform(sg[t]);
   }
   auto gval = [&](int t) -> T { return gk[t]; };
-  const T w = MODE >= TM_JAC ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
+  const T w = kJac ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
+  double dot = 0.0;  // TM_YDOT: this thread's share of x.y
 
   // element slots: slice s = wave + 8 m -> column s / spc, z range 64 (s % spc) + [0, 64)
   int scl[kWSlots], sez[kWSlots];
@@ -326,8 +329,8 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
           cq[q][r] = cp[(dd[q] + r) & cmask];
-          bq[q][r] = MODE != TM_Y ? b[dd[q] + r] : T(0);
-          dq[q][r] = MODE >= TM_JAC ? Dinv[dd[q] + r] : T(0);
+          bq[q][r] = !kProd ? b[dd[q] + r] : T(0);
+          dq[q][r] = kJac ? Dinv[dd[q] + r] : T(0);
           aq[q][r] = MODE == TM_JAC_ADD ? xadd[dd[q] + r] : T(0);
         }
       }
@@ -427,13 +430,14 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
               if (cr) xv = x[d];
               const T ax = cr ? xv : ax0;
               T vv;
-              if (MODE == TM_Y)
+              if (kProd)
                 vv = ax;
               else if (MODE == TM_RES)
                 vv = bq[q][r] - ax;
               else
                 vv = xv + w * dq[q][r] * (bq[q][r] - ax);
               if (MODE == TM_JAC_ADD) vv += aq[q][r];
+              if (MODE == TM_YDOT) dot += (double)xv * (double)vv;
               y[d] = vv;
             }
           }
@@ -444,6 +448,22 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
         store_e();
       }
       __syncthreads();
+    }
+  }
+  if (MODE == TM_YDOT) {
+    // fixed-order workgroup sum: waves, then wave partials in index order; the
+    // wave partials reuse the (now idle) dynamic LDS, so the kernel's static LDS
+    // stays zero and the full dynamic allowance remains available
+    double* wsum = reinterpret_cast<double*>(lds_raw);
+    __syncthreads();
+    dot = wave_sum(dot);
+    if (lane == 0) wsum[wave] = dot;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+#pragma unroll
+      for (int v = 0; v < NT / 64; ++v) s += wsum[v];
+      partial[blockIdx.x] = s;
     }
   }
 }
@@ -483,25 +503,60 @@ static int wht_device_cus() {
 template <typename T, int MODE, int NT>
 static void launch_wht_nt(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const T* K0, const T* E,
                           const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
-                          double wscale, const T* xadd, T* y) {
+                          double wscale, const T* xadd, T* y, double* partial) {
   static bool attr_set = false;
   if (!attr_set) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&op0_wht_kernel<T, MODE, NT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWMaxLds);
     attr_set = true;
   }
-  op0_wht_kernel<T, MODE, NT><<<n_wg, NT, lds, s>>>(A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y);
+  op0_wht_kernel<T, MODE, NT><<<n_wg, NT, lds, s>>>(A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial);
 }
 
 template <typename T, int MODE>
 static void launch_wht(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const T* K0, const T* E,
                        const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
-                       double wscale, const T* xadd, T* y) {
+                       double wscale, const T* xadd, T* y, double* partial = nullptr) {
   if (g_wht_nt[sizeof(T) == 8 ? 1 : 0] == 1024)
-    launch_wht_nt<T, MODE, 1024>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y);
+    launch_wht_nt<T, MODE, 1024>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial);
   else
-    launch_wht_nt<T, MODE, 512>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y);
+    launch_wht_nt<T, MODE, 512>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial);
 }
+
+static MxArgs wht_args(const Grid& g, int64_t e_sx, int64_t e_sy, int* n_wg) {
+  MxArgs A;
+  A.g = g;
+  A.e_sx = e_sx;
+  A.e_sy = e_sy;
+  const int tc = wht_tc(g);
+  A.n_xt = (g.nx() + tc - 2) / (tc - 1);
+  A.nzg = 0;
+  A.units = (int64_t)A.n_xt * g.ny();
+  const int64_t slots = (int64_t)wht_device_cus() * g_wht_wgs;
+  *n_wg = (int)(A.units < slots ? A.units : slots);
+  return A;
+}
+
+int wht_ydot_partials(const Grid& g) {
+  int n_wg = 0;
+  (void)wht_args(g, 0, 0, &n_wg);
+  return n_wg;
+}
+
+template <typename T>
+int op0_wht_ydot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E, const uint8_t* cons,
+                 const T* x, T* y, double* partial, int max_partial, hipStream_t s) {
+  ET_REQUIRE(wht_supported(g, sizeof(T)), "transformed-basis operator: 3-D grids with nelz <= 128 only");
+  int n_wg = 0;
+  const MxArgs A = wht_args(g, e_sx, e_sy, &n_wg);
+  ET_REQUIRE(partial != nullptr && n_wg <= max_partial, "transformed-basis operator: partial buffer too small");
+  const size_t lds = wht_lds_values(g) * sizeof(T);
+  launch_wht<T, TM_YDOT>(n_wg, lds, s, A, K0, E, cons, x, nullptr, nullptr, nullptr, 1.0, nullptr, y, partial);
+  ET_CHECK_LAUNCH();
+  return ET_OK;
+}
+template int op0_wht_ydot<double>(const Grid&, int64_t, int64_t, const double*, const double*, const uint8_t*,
+                                  const double*, double*, double*, int, hipStream_t);
 
 template <typename T>
 int op0_wht(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E,

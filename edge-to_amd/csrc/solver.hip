@@ -321,7 +321,22 @@ __global__ __launch_bounds__(256) void pcg_update_p_kernel(int64_t n, const doub
                                                            const TZ* __restrict__ z,
                                                            double* __restrict__ p, int first) {
   const double beta = first ? 0.0 : sc[SC_RHO_NEW] / sc[SC_RHO];
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+  // four consecutive values per step (z and p are 16-byte aligned allocations)
+  const int64_t n4 = n >> 2;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    TZ zv[4];
+    double pv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) zv[u] = z[4 * t + u];
+    if (!first) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pv[u] = p[4 * t + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) p[4 * t + u] = first ? (double)zv[u] : (double)zv[u] + beta * pv[u];
+  }
+  for (int64_t t = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x)
     p[t] = first ? (double)z[t] : (double)z[t] + beta * p[t];
 }
@@ -846,6 +861,10 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
   pcg_update_p_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, z, p_, 1);
   prof.end(s);
   ET_CHECK_LAUNCH();
+  // the fused q = A p, p.q path exists for the transformed-basis operator
+  const bool fused_pq = sh.tiled && g0.dim == 3 && get_wht() && wht_supported(g0, sizeof(double)) &&
+                        wht_ydot_partials(g0) <= kRedBlocks;
+  const int n_pq = fused_pq ? wht_ydot_partials(g0) : 0;
   int last = 0;
   auto read_ring = [&](int k, double* R, int* st) -> bool {
     ET_CHECK_HIP(hipEventSynchronize(ring_ev[k & 1]));
@@ -858,9 +877,19 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
   int status = ET_OK;
   bool stopped = false;
   for (int it = 1; it <= maxiter; ++it) {
-    ET_TRY(apply_A64(p_, q, OP_Y, nullptr));
-    prof.begin(ET_PROF_PCG_VEC, s);
-    ET_TRY(dot(n_dof, p_, q, partial, sh.scal + SC_PQ, s));
+    if (fused_pq) {
+      // q = A p with p.q summed per workgroup inside the operator
+      prof.begin(ET_PROF_MATVEC64, s);
+      ET_TRY(op0_wht_ydot<double>(g0, (int64_t)g0.ey * g0.ez, g0.ez, sh.K0d, sh.Ed, sh.cons, p_, q, partial,
+                                  kRedBlocks, s));
+      prof.end(s);
+      prof.begin(ET_PROF_PCG_VEC, s);
+      ET_TRY(reduce_finalize_n(partial, n_pq, sh.scal + SC_PQ, s));
+    } else {
+      ET_TRY(apply_A64(p_, q, OP_Y, nullptr));
+      prof.begin(ET_PROF_PCG_VEC, s);
+      ET_TRY(dot(n_dof, p_, q, partial, sh.scal + SC_PQ, s));
+    }
     pcg_update_xr_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, p_, q, x, r, rb, partial);
     ET_CHECK_LAUNCH();
     ET_TRY(reduce_finalize(partial, sh.scal + SC_RR, s));

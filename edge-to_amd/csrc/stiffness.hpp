@@ -37,6 +37,12 @@ int op0_wht(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, co
             const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
             double wscale, const T* xadd, T* y, hipStream_t s);
 bool wht_supported(const Grid& g, size_t elem_bytes);
+// y = A x (FP64) fused with the x.y partial sum of every workgroup (fixed order);
+// writes wht_ydot_partials(g) <= max_partial values to partial.
+template <typename T>
+int op0_wht_ydot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E, const uint8_t* cons,
+                 const T* x, T* y, double* partial, int max_partial, hipStream_t s);
+int wht_ydot_partials(const Grid& g);
 void set_wht(int on);
 int get_wht();
 void set_wht_wgs(int n);
