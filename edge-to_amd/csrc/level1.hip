@@ -17,11 +17,13 @@
 // values, 279 + 150 FMAs, one inverse transform.
 //
 // Passes per application, no atomics, fixed summation order:
-//   1. one thread per coarse element writes its 8 corner results (24 values,
-//      stored component-plane by component-plane for coalescing);
-//   2. one thread per coarse node sums the results of its (up to 8) elements,
-//      scales by 1/tw and applies the mode epilogue (y = Ax, b - Ax, Jacobi,
-//      Jacobi + add); nodes with a correction row store the raw sum instead;
+//   1. one thread per coarse element computes its 8 corner results; the top
+//      (bz = 1) ones move one lane up to the element above, so each element
+//      stores the 4 node sums of its bottom face (12 values, plane by plane);
+//   2. one thread per coarse node sums the bottom-face sums of its (up to 4)
+//      element columns, scales by 1/tw and applies the mode epilogue (y = Ax,
+//      b - Ax, Jacobi, Jacobi + add); nodes with a correction row store the
+//      raw sum instead;
 //   3. one thread per (corrected node, component) adds its 27 correction
 //      blocks and applies the epilogue.
 #include <cmath>
@@ -178,25 +180,42 @@ __global__ void l1_convert_kernel(int64_t n, const TI* __restrict__ a, TO* __res
 
 // ------------------------------------------------------------ apply ----
 
+// lane j takes lane j-1's value (lane 0 takes 0)
+__device__ __forceinline__ float l1_shr(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0u, __builtin_bit_cast(unsigned, v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double l1_shr(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)u, 0x138, 0xf, 0xf, false);
+  const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(u >> 32), 0x138, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Element pass.  Thread = coarse element C (lane = C mod 64).  The bz = 1
+// corner results of element C-1 move one lane up and are added to the bz = 0
+// results of C when both lie in one column and one wavefront, so lo holds per
+// element the 4 node sums (bx, by) x 3 of its bottom face; results that cannot
+// move (lane 63, the top element of a column) are stored in top instead.
+// Plane (bx + 2 by) * 3 + c of lo / top, element index C.
 template <typename T>
 __global__ __launch_bounds__(256) void l1_elem_kernel(const Grid g1, const T* __restrict__ cf,
                                                       const T* __restrict__ Eh, const T* __restrict__ x,
-                                                      T* __restrict__ out) {
-  __shared__ T scf[kL1Terms];
-  for (int t = threadIdx.x; t < kL1Terms; t += blockDim.x) scf[t] = cf[t];
-  __syncthreads();
-  const int C = blockIdx.x * blockDim.x + threadIdx.x;
-  if (C >= (int)g1.n_elem()) return;
+                                                      T* __restrict__ lo, T* __restrict__ top) {
+  const int ne = (int)g1.n_elem();
+  const int C0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = C0 < ne;
+  const int C = valid ? C0 : ne - 1;  // every lane takes part in the lane shift
+  const int lane = threadIdx.x & 63;
   const int K = C % g1.ez, IJ = C / g1.ez, J = IJ % g1.ey, I = IJ / g1.ey;
   const int nx = g1.nx(), nz = g1.nz();
   T xh[3][8];
 #pragma unroll
   for (int beta = 0; beta < 8; ++beta) {
     const int i = I + (beta & 1), j = J + ((beta >> 1) & 1), k = K + (beta >> 2);
-    const int64_t n = ((int64_t)j * nx + i) * nz + k;
-    const T s = l1_inv_tw_t<T>(g1, i, j, k);
+    const int n = (j * nx + i) * nz + k;
+    const T sw = l1_inv_tw_t<T>(g1, i, j, k);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) xh[c][beta] = x[n * 3 + c] * s;
+    for (int c = 0; c < 3; ++c) xh[c][beta] = x[n * 3 + c] * sw;
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) l1_wht8(xh[c]);
@@ -204,18 +223,33 @@ __global__ __launch_bounds__(256) void l1_elem_kernel(const Grid g1, const T* __
 #pragma unroll
   for (int m = 0; m < 8; ++m) eh[m] = Eh[(int64_t)C * 8 + m];
   T yh[3][8];
-  level1_product(xh, eh, [&](int k) -> T { return scf[k]; }, yh);
+  // the coefficients are wave-uniform: scalar loads into SGPRs (staging them
+  // in LDS lets the compiler hoist every read and quadruples the VGPRs)
+  level1_product(xh, eh, [&](int k) -> T { return cf[k]; }, yh);
 #pragma unroll
   for (int c = 0; c < 3; ++c) l1_wht8(yh[c]);
-  const int64_t ne = g1.n_elem();
+  const bool from_prev = lane > 0 && K > 0;          // lane-1 holds C-1 of this column
+  const bool keep_top = lane == 63 || K == g1.ez - 1;  // C's bz = 1 results stay here
 #pragma unroll
-  for (int beta = 0; beta < 8; ++beta)
+  for (int f = 0; f < 4; ++f)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) out[(beta * 3 + c) * ne + C] = yh[c][beta];
+    for (int c = 0; c < 3; ++c) {
+      const T up = yh[c][f + 4];
+      const T moved = l1_shr(up);
+      if (valid) {
+        lo[(int64_t)(f * 3 + c) * ne + C] = yh[c][f] + (from_prev ? moved : T(0));
+        if (keep_top) top[(int64_t)(f * 3 + c) * ne + C] = up;
+      }
+    }
 }
 
+// Node pass: sum of the (up to 4) element columns' bottom-face sums, plus the
+// unmoved top results of the element below where there are any; scale by
+// 1/tw; epilogue.  Nodes with a correction row store the raw sum (finished by
+// l1_fix_kernel).
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void l1_node_kernel(const Grid g1, const T* __restrict__ part,
+__global__ __launch_bounds__(256) void l1_node_kernel(const Grid g1, const T* __restrict__ lo,
+                                                      const T* __restrict__ top,
                                                       const int* __restrict__ aidx, int n_aff,
                                                       const T* __restrict__ dA, const T* __restrict__ x,
                                                       const T* __restrict__ b, const T* __restrict__ Dinv,
@@ -223,20 +257,28 @@ __global__ __launch_bounds__(256) void l1_node_kernel(const Grid g1, const T* __
                                                       const T* __restrict__ xadd, T* __restrict__ y) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (int)g1.n_nodes()) return;
-  const int nx = g1.nx(), ny = g1.ny(), nz = g1.nz();
+  const int nx = g1.nx(), nz = g1.nz();
   const int k = n % nz, ij = n / nz, i = ij % nx, j = ij / nx;
+  const int ne = (int)g1.n_elem();
   T acc[3] = {T(0), T(0), T(0)};
-  const int64_t ne = g1.n_elem();
-  // branch-free: missing elements read element 0 with weight zero
 #pragma unroll
-  for (int beta = 0; beta < 8; ++beta) {
-    const int I = i - (beta & 1), J = j - ((beta >> 1) & 1), K = k - (beta >> 2);
-    const bool in = I >= 0 && I < g1.ex && J >= 0 && J < g1.ey && K >= 0 && K < g1.ez;
-    const int64_t C = in ? (int64_t)(I * g1.ey + J) * g1.ez + K : 0;
+  for (int f = 0; f < 4; ++f) {
+    const int I = i - (f & 1), J = j - (f >> 1);
+    const bool col = I >= 0 && I < g1.ex && J >= 0 && J < g1.ey;
+    const int Cb = (I * g1.ey + J) * g1.ez;  // first element of the column
+    // element k of the column: its bottom face (with C-1's moved results)
+    const bool in = col && k < g1.ez;
+    const int C = in ? Cb + k : 0;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      const T v = part[(beta * 3 + r) * ne + C];
+      const T v = lo[(int64_t)(f * 3 + r) * ne + C];
       acc[r] += in ? v : T(0);
+    }
+    // element k-1 whose top results were not moved (rare: lane 63 or column top)
+    const int Cm = Cb + k - 1;
+    if (col && k > 0 && ((Cm & 63) == 63 || k == g1.ez)) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) acc[r] += top[(int64_t)(f * 3 + r) * ne + Cm];
     }
   }
   const T s = l1_inv_tw_t<T>(g1, i, j, k);
@@ -407,13 +449,15 @@ int Level1MF::apply(int mode, const T* x, const T* b, const T* Dinv, const doubl
                     const T* xadd, T* y, hipStream_t s) const {
   ET_REQUIRE((sizeof(T) == sizeof(float)) == mixed, "level-1 operator: precision mismatch");
   const T* dAt = mixed ? (const T*)dA : (const T*)dA64;
-  l1_elem_kernel<T><<<grid_for(g1.n_elem()), 256, 0, s>>>(g1, (const T*)cf, (const T*)Eh, x, (T*)scratch);
+  T* lo = (T*)scratch;
+  T* top = lo + 12 * g1.n_elem();
+  l1_elem_kernel<T><<<grid_for(g1.n_elem()), 256, 0, s>>>(g1, (const T*)cf, (const T*)Eh, x, lo, top);
   const unsigned gn = grid_for(g1.n_nodes());
   const unsigned gf = grid_for((int64_t)n_aff * 3);
 #define ET_L1(M)                                                                                     \
   do {                                                                                               \
-    l1_node_kernel<T, M><<<gn, 256, 0, s>>>(g1, (const T*)scratch, aidx, n_aff, dAt, x, b, Dinv, omega, \
-                                            wscale, xadd, y);                                        \
+    l1_node_kernel<T, M><<<gn, 256, 0, s>>>(g1, lo, top, aidx, n_aff, dAt, x, b, Dinv, omega, wscale, \
+                                            xadd, y);                                                \
     if (n_aff > 0)                                                                                   \
       l1_fix_kernel<T, M><<<gf, 256, 0, s>>>(g1, alist, n_aff, dAt, x, b, Dinv, omega, wscale, xadd, y); \
   } while (0)

@@ -12,7 +12,7 @@ struct Level1MF {
   bool mixed = false;         // FP32 apply (else FP64)
   void* cf = nullptr;         // element coefficients (kL1Terms, working precision)
   void* Eh = nullptr;         // per coarse element: transform of its 8 children's E [n_elem1][8]
-  void* scratch = nullptr;    // per coarse element: corner results [n_elem1][24]
+  void* scratch = nullptr;    // per coarse element: bottom-face node sums [12][n_elem1], unmoved top results [12][n_elem1]
   uint8_t* special = nullptr; // per coarse element: a child touches a constrained DOF
   int* aidx = nullptr;        // per coarse node: row in the correction list, or -1
   int* alist = nullptr;       // corrected coarse nodes

@@ -419,10 +419,13 @@ int op0_fine(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, c
              double wscale, const T* xadd, T* y, hipStream_t s) {
   if (get_wht() && wht_supported(g, sizeof(T)))
     return op0_wht<T>(mode, g, e_sx, e_sy, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, s);
+  ET_REQUIRE(mode >= 0 && mode <= 3, "fine operator: mode not available without the transformed-basis operator");
   if (get_mx(sizeof(T) == 8) && mx_supported(g, sizeof(T)))
     return op0_mx<T>(mode, g, e_sx, e_sy, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, s);
   return op0_tiled<T>(mode, g, e_sx, e_sy, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, s);
 }
+
+bool fine_jac0_supported(const Grid& g, size_t elem_bytes) { return get_wht() && wht_supported(g, elem_bytes); }
 
 template int op0_fine<double>(int, const Grid&, int64_t, int64_t, const double*, const double*,
                               const uint8_t*, const double*, const double*, const double*,

@@ -125,7 +125,8 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
                                                             double wscale, const T* __restrict__ xadd,
                                                             T* __restrict__ y, double* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  constexpr bool kJac = MODE == TM_JAC || MODE == TM_JAC_ADD;
+  constexpr bool kJac = MODE == TM_JAC || MODE == TM_JAC_ADD || MODE == TM_JAC0;
+  constexpr bool kZero = MODE == TM_JAC0;  // input x0 = w D^-1 b, built while staging
   constexpr bool kProd = MODE == TM_Y || MODE == TM_YDOT;  // output is A x
   const Grid& g = A.g;
   const int nx = g.nx(), ny = g.ny(), nz = g.nz();
@@ -226,6 +227,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
     // masks are applied when the registers are written to LDS, so a layer's loads
     // are all in flight together instead of each waiting on its constraint byte.
     T pu[kWStage][3];
+    T pdv[kZero ? kWStage : 1][3];  // TM_JAC0: D^-1 of the staged nodes
     uint8_t pk[kWStage][3];
     bool pv[kWStage];
     T pe[kWEStage];
@@ -242,7 +244,12 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
           const int d = pv[m] ? pb + s * 3 : 0;
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
-            pu[m][r] = x[d + r];
+            if (kZero) {
+              pu[m][r] = b[d + r];
+              pdv[kZero ? m : 0][r] = Dinv[d + r];
+            } else {
+              pu[m][r] = x[d + r];
+            }
             pk[m][r] = cp[(d + r) & cmask];
           }
         }
@@ -255,7 +262,10 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
         const int s = tid + kWThreads * m;
         if (s < span) {
 #pragma unroll
-          for (int r = 0; r < 3; ++r) dst[r * span + s] = (pv[m] && pk[m][r] == 0) ? pu[m][r] : T(0);
+          for (int r = 0; r < 3; ++r) {
+            const T v = kZero ? w * pdv[kZero ? m : 0][r] * pu[m][r] : pu[m][r];
+            dst[r * span + s] = (pv[m] && pk[m][r] == 0) ? v : T(0);
+          }
         }
       }
     };
@@ -399,7 +409,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
       }
       // the node's own (staged, constrained-zeroed) value is read before the
       // barrier: after it, the slot of plane ey is refilled by other threads
-      if (MODE != TM_Y) {
+      if (MODE != TM_Y && !kZero) {
         const T* own = su + (size_t)(ey & 1) * 3 * span;
 #pragma unroll
         for (int q = 0; q < kWNodes; ++q) {
@@ -426,8 +436,13 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
               if (seam)
                 ax0 += ss[((r * 2) * tc + nl) * 2 + (k >> 6) - 1] + ss[((r * 2 + 1) * tc + nl - 1) * 2 + (k >> 6) - 1];
               const bool cr = cq[q][r] != 0;
-              T xv = xq[q][r];  // zero on constrained DOFs
-              if (cr) xv = x[d];
+              T xv;
+              if (kZero) {
+                xv = w * dq[q][r] * bq[q][r];  // x0 of this DOF (constrained or not)
+              } else {
+                xv = xq[q][r];  // zero on constrained DOFs
+                if (cr) xv = x[d];
+              }
               const T ax = cr ? xv : ax0;
               T vv;
               if (kProd)
@@ -578,7 +593,9 @@ int op0_wht(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, co
     case TM_Y: launch_wht<T, TM_Y>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
     case TM_RES: launch_wht<T, TM_RES>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
     case TM_JAC: launch_wht<T, TM_JAC>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
-    default: launch_wht<T, TM_JAC_ADD>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
+    case TM_JAC_ADD: launch_wht<T, TM_JAC_ADD>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
+    case TM_JAC0: launch_wht<T, TM_JAC0>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
+    default: ET_REQUIRE(false, "transformed-basis operator: unknown mode");
   }
   ET_CHECK_LAUNCH();
   return ET_OK;

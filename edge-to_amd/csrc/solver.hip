@@ -24,7 +24,9 @@ namespace et {
 
 // ------------------------------------------------------------ kernels ----
 
-enum OpMode { OP_Y = 0, OP_RES = 1, OP_JAC = 2, OP_JAC_ADD = 3 };
+// OP_JAC0: first Jacobi sweep from the zero guess fused with it (fine grid,
+// transformed-basis operator only); x is not read
+enum OpMode { OP_Y = 0, OP_RES = 1, OP_JAC = 2, OP_JAC_ADD = 3, OP_JAC0 = 5 };
 
 template <typename T, int DIM, int MODE>
 __device__ __forceinline__ void op_epilogue(int64_t n, const T (&ax)[DIM],
@@ -473,11 +475,20 @@ int MGHierarchy<T>::vcycle(int l, MGShared& sh, hipStream_t s) {
   if (n_smooth == 0) {
     ET_CHECK_HIP(hipMemsetAsync(L.x, 0, n * sizeof(T), s));
   } else {
-    sh.prof->begin(l == 0 ? ET_PROF_L0_SCALE : ET_PROF_COARSE_OPS, s);
-    scale_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, L.Dinv, L.b, om, ws, L.x);
-    sh.prof->end(s);
-    ET_CHECK_LAUNCH();
-    for (int it = 1; it < n_smooth; ++it) {
+    int it = 1;
+    // FP32 only: the FP64 operator is register-bound and runs the two passes faster
+    if (l == 0 && sizeof(T) == 4 && n_smooth >= 2 && sh.tiled && !sh.mfma && fine_jac0_supported(L.g, sizeof(T))) {
+      // x0 = w D^-1 b and the first sweep from it in one pass
+      ET_TRY(op_launch<T>(L, sh, OP_JAC0, nullptr, L.b, om, ws, nullptr, L.t, s));
+      std::swap(L.x, L.t);
+      it = 2;
+    } else {
+      sh.prof->begin(l == 0 ? ET_PROF_L0_SCALE : ET_PROF_COARSE_OPS, s);
+      scale_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, L.Dinv, L.b, om, ws, L.x);
+      sh.prof->end(s);
+      ET_CHECK_LAUNCH();
+    }
+    for (; it < n_smooth; ++it) {
       ET_TRY(op_launch<T>(L, sh, OP_JAC, L.x, L.b, om, ws, nullptr, L.t, s));
       std::swap(L.x, L.t);
     }

@@ -55,6 +55,8 @@ template <typename T>
 int op0_fine(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E,
              const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
              double wscale, const T* xadd, T* y, hipStream_t s);
+// whether op0_fine supports the fused zero-guess Jacobi mode (5) on this grid
+bool fine_jac0_supported(const Grid& g, size_t elem_bytes);
 bool fine_supported(const Grid& g, size_t elem_bytes);
 // Any level-0 operator mode on any grid (fine-grid kernels in 3-D when supported).
 // omega == nullptr means omega = 1.
