@@ -90,6 +90,7 @@ int et_tune_set(int param, int value) {
     } else if (param == ET_TUNE_LEVEL1_MF) {
       et::set_level1_mf(value);
     } else if (param == ET_TUNE_COARSE_LIB) {
+      ET_REQUIRE(value >= 0 && value <= 2, "et_tune_set: coarse factor path must be 0, 1 or 2");
       et::set_coarse_lib(value);
     } else if (param == ET_TUNE_WHT) {
       et::set_wht(value);
@@ -417,6 +418,14 @@ int et_mg_profile(et_mg* handle, int enable) {
     handle->impl.prof.reset();
     handle->impl.prof.mask = (unsigned)enable & ((1u << ET_PROF_KINDS) - 1u);
     return ET_OK;
+  })
+}
+
+int et_mg_coarse_solve(et_mg* handle, const void* b, void* x) {
+  ET_GUARD({
+    ET_REQUIRE(handle && b && x, "et_mg_coarse_solve: null pointer");
+    ET_REQUIRE(b != x, "et_mg_coarse_solve: b and x must not alias");
+    return handle->impl.coarse_solve(b, x);
   })
 }
 

@@ -1,15 +1,19 @@
 // Coarsest-grid direct solve in FP64: the coarsest Galerkin stencil is
 // expanded to a dense SPD matrix, Cholesky-factored and inverted once per
-// solve, so that every V-cycle's coarse solve is one dense symmetric
-// matrix-vector product (one wavefront per row, coalesced).  Two factor
-// paths: rocSOLVER's potrf + potri (default), or the in-house blocked
-// right-looking Cholesky (64x64 tiles in LDS) with A^-1 = L^-T L^-1.
+// solve.  Three factor paths: the band Cholesky (in-house tile kernels,
+// rocBLAS band updates) with L^-1 from rocSOLVER trtri (default; each
+// V-cycle's coarse solve is two triangular products), rocSOLVER potrf +
+// potri, or the in-house dense blocked Cholesky with A^-1 = L^-T L^-1 (one
+// dense symmetric product per coarse solve for the last two).
 // The matrix is padded to a multiple of 64 with an identity block.
+#include <algorithm>
+
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
 #include "common.hpp"
 #include "coarse.hpp"
+#include "chol_tile.hpp"
 
 namespace et {
 
@@ -229,6 +233,101 @@ __global__ __launch_bounds__(256) void yty_kernel(const double* Y, double* Ainv,
     }
 }
 
+// ------------------------------ band Cholesky + rocSOLVER triangular inverse ---
+// Column-major view (the matrix is symmetric, so the dense row-major matrix
+// is also its column-major self), 64x64 tiles, L lower.  The coarse matrix
+// is banded: with node order (j nx + i) nz + k its half-bandwidth is
+// b = dof (nx nz + nz + 1) + dof - 1, so tile (i, m) of A and of L is zero
+// for i - m > nbd = ceil(b / 64), and each factor step k touches only the nr
+// = min(nbd, tiles left) tiles below the diagonal:
+//   diag:   one wavefront factors A_kk and forms W_kk = L_kk^-1 (chol_tile.hpp);
+//   panel:  A_ik <- A_ik W_kk^T for the nr tiles below;
+//   update: A_22 -= L_21 L_21^T on the nr x nr band block (rocBLAS dsyrk).
+// rocSOLVER trtri then overwrites L with Y = L^-1, and the coarse solve is
+// x = Y^T (Y b): two triangular products reading n^2 / 2 values each.
+
+__global__ __launch_bounds__(64) void bchol_diag_kernel(double* A, int ld, int o, double* Wk, int* info) {
+  chol_tile64(A, ld, o, Wk, info);
+}
+
+// panel: tiles i = kb+1 .. kb+nr: A_ik <- A_ik W^T
+__global__ __launch_bounds__(256) void bchol_panel_kernel(double* A, int ld, int kb, const double* Wk) {
+  __shared__ double Xs[NB][NB + 1];  // [row][m]
+  __shared__ double Ws[NB][NB + 1];  // [c][m] = W(c, m)
+  const int o = kb * NB, row0 = o + NB * (1 + blockIdx.x);
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    const int rr = t % NB, cc = t / NB;  // consecutive threads, consecutive rows (column-major)
+    Xs[rr][cc] = A[(int64_t)(o + cc) * ld + row0 + rr];
+    Ws[rr][cc] = Wk[rr + cc * NB];
+  }
+  __syncthreads();
+  const int r0 = (threadIdx.x % 16) * 4, c0 = (threadIdx.x / 16) * 4;
+  double acc[4][4] = {};
+  for (int m = 0; m < NB; ++m) {
+    double xa[4], wb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xa[u] = Xs[r0 + u][m];
+      wb[u] = Ws[c0 + u][m];
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[p][q] += xa[p] * wb[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) A[(int64_t)(o + c0 + q) * ld + row0 + r0 + p] = acc[p][q];
+}
+
+// Z = Y row-major (Z[i ld + j] = Y_ij) for the tiles on and below the diagonal
+// (above the diagonal of the diagonal tiles: zero).  One workgroup per tile.
+__global__ __launch_bounds__(256) void lower_transpose_kernel(const double* __restrict__ Y, int ld,
+                                                              double* __restrict__ Z) {
+  __shared__ double Ts[NB][NB + 1];
+  int ib, jb;
+  tri_index(blockIdx.x, ib, jb);
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    const int rr = t % NB, cc = t / NB;
+    const double v = Y[(int64_t)(jb * NB + cc) * ld + ib * NB + rr];
+    Ts[rr][cc] = (ib == jb && cc > rr) ? 0.0 : v;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    const int cc = t % NB, rr = t / NB;  // consecutive threads, consecutive columns of a Z row
+    Z[(int64_t)(ib * NB + rr) * ld + jb * NB + cc] = Ts[rr][cc];
+  }
+}
+
+// t_i = sum_{j <= i} Y_ij b_j  (row i of Y = row i of Z); one wavefront per row
+template <typename TI>
+__global__ __launch_bounds__(256) void trmv_lower_kernel(const double* __restrict__ Z, int ld, int n,
+                                                         const TI* __restrict__ b, double* __restrict__ t) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const double* z = Z + (int64_t)i * ld;
+  double s = 0.0;
+  for (int j = lane; j <= i; j += 64) s += z[j] * (double)b[j];
+  s = wave_sum(s);
+  if (lane == 0) t[i] = s;
+}
+
+// x_j = sum_{i >= j} Y_ij t_i  (column j of Y); one wavefront per column
+template <typename TO>
+__global__ __launch_bounds__(256) void trmv_upper_kernel(const double* __restrict__ Y, int ld, int n,
+                                                         const double* __restrict__ t, TO* __restrict__ x) {
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (j >= n) return;
+  const double* y = Y + (int64_t)j * ld;
+  double s = 0.0;
+  for (int i = j + lane; i < n; i += 64) s += y[i] * t[i];
+  s = wave_sum(s);
+  if (lane == 0) x[j] = (TO)s;
+}
+
 // -------------------------------------------------------- coarse solve ---
 
 // x = Ainv b (first n entries); one wavefront per row.
@@ -253,18 +352,24 @@ __global__ __launch_bounds__(256) void mirror_upper_kernel(double* A, int ld, in
   for (int j = threadIdx.x; j < i && i < n; j += blockDim.x) A[(int64_t)i * ld + j] = src[(int64_t)j * ld];
 }
 
-static int g_coarse_lib = 1;
-void set_coarse_lib(int on) { g_coarse_lib = on ? 1 : 0; }
+static int g_coarse_lib = 2;
+void set_coarse_lib(int mode) { g_coarse_lib = mode >= 0 && mode <= 2 ? mode : 2; }
 int get_coarse_lib() { return g_coarse_lib; }
 
 int CoarseSolver::init(int n_, hipStream_t s) {
   n = n_;
   ld = (n + NB - 1) / NB * NB;
   nbt = ld / NB;
-  use_lib = g_coarse_lib != 0;
+  lib_mode = g_coarse_lib;
+  use_lib = lib_mode != 0;
   const size_t bytes = (size_t)ld * ld * sizeof(double);
   ET_CHECK_HIP(hipMalloc(&A, bytes));
-  if (!use_lib) ET_CHECK_HIP(hipMalloc(&Y, bytes));
+  if (lib_mode == 0) ET_CHECK_HIP(hipMalloc(&Y, bytes));
+  if (lib_mode == 2) {
+    ET_CHECK_HIP(hipMalloc(&Z, bytes));
+    ET_CHECK_HIP(hipMalloc(&Wd, (size_t)NB * NB * sizeof(double)));
+    ET_CHECK_HIP(hipMalloc(&tv, (size_t)ld * sizeof(double)));
+  }
   ET_CHECK_HIP(hipMalloc(&info, 2 * sizeof(int)));
   ET_CHECK_HIP(hipHostMalloc(&info_h, 2 * sizeof(int)));
   if (use_lib) {
@@ -281,6 +386,10 @@ void CoarseSolver::release() {
   blas = nullptr;
   if (A) (void)hipFree(A);
   if (Y) (void)hipFree(Y);
+  if (Wd) (void)hipFree(Wd);
+  if (tv) (void)hipFree(tv);
+  if (Z) (void)hipFree(Z);
+  Wd = tv = Z = nullptr;
   if (info) (void)hipFree(info);
   if (info_h) (void)hipHostFree(info_h);
   A = Y = nullptr;
@@ -291,7 +400,7 @@ void CoarseSolver::release() {
 int CoarseSolver::factor(const Grid& g, int dof, const double* S, hipStream_t s) {
   const size_t bytes = (size_t)ld * ld * sizeof(double);
   ET_CHECK_HIP(hipMemsetAsync(A, 0, bytes, s));
-  if (Y) ET_CHECK_HIP(hipMemsetAsync(Y, 0, bytes, s));
+  if (Y && lib_mode == 0) ET_CHECK_HIP(hipMemsetAsync(Y, 0, bytes, s));
   ET_CHECK_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int), s));
   const int NO = g.dim == 3 ? 27 : 9;
   const unsigned grid = grid_for(g.n_nodes() * NO);
@@ -303,6 +412,36 @@ int CoarseSolver::factor(const Grid& g, int dof, const double* S, hipStream_t s)
   if (ld > n) {
     pad_identity_kernel<<<grid_for(ld - n), kBlock, 0, s>>>(n, ld, A);
     ET_CHECK_LAUNCH();
+  }
+  if (lib_mode == 2) {
+    rocblas_handle h = static_cast<rocblas_handle>(blas);
+    ET_REQUIRE(rocblas_set_stream(h, s) == rocblas_status_success, "rocblas_set_stream failed");
+    // half-bandwidth of the node ordering (j nx + i) nz + k, in tiles
+    const int64_t bw = g.dim == 3 ? (int64_t)dof * ((int64_t)g.nx() * g.nz() + g.nz() + 1) + dof - 1
+                                  : (int64_t)dof * (g.nx() + 1) + dof - 1;
+    const int nbd = (int)std::min<int64_t>((bw + NB - 1) / NB, nbt - 1);
+    const double minus1 = -1.0, one = 1.0;
+    for (int kb = 0; kb < nbt; ++kb) {
+      const int nr = std::min(nbd, nbt - kb - 1);
+      bchol_diag_kernel<<<1, 64, 0, s>>>(A, ld, kb * NB, Wd, info);
+      ET_CHECK_LAUNCH();
+      if (nr > 0) {
+        bchol_panel_kernel<<<nr, 256, 0, s>>>(A, ld, kb, Wd);
+        ET_CHECK_LAUNCH();
+        const int o = kb * NB, m = nr * NB;
+        ET_REQUIRE(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, m, NB, &minus1,
+                                 A + (int64_t)o * ld + o + NB, ld, &one, A + (int64_t)(o + NB) * ld + o + NB,
+                                 ld) == rocblas_status_success,
+                   "rocblas_dsyrk failed");
+      }
+    }
+    ET_REQUIRE(rocsolver_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_non_unit, ld, A, ld, info + 1) ==
+                   rocblas_status_success,
+               "rocsolver_dtrtri failed");
+    lower_transpose_kernel<<<nbt * (nbt + 1) / 2, 256, 0, s>>>(A, ld, Z);
+    ET_CHECK_LAUNCH();
+    ET_CHECK_HIP(hipMemcpyAsync(info_h, info, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    return ET_OK;
   }
   if (use_lib) {
     rocblas_handle h = static_cast<rocblas_handle>(blas);
@@ -354,6 +493,13 @@ int CoarseSolver::check_info(hipStream_t s) {
 
 template <typename TI, typename TO>
 int CoarseSolver::solve(const TI* b, TO* x, hipStream_t s) const {
+  if (lib_mode == 2) {
+    trmv_lower_kernel<TI><<<(n + 3) / 4, 256, 0, s>>>(Z, ld, n, b, tv);
+    ET_CHECK_LAUNCH();
+    trmv_upper_kernel<TO><<<(n + 3) / 4, 256, 0, s>>>(A, ld, n, tv, x);
+    ET_CHECK_LAUNCH();
+    return ET_OK;
+  }
   symv_kernel<TI, TO><<<(n + 3) / 4, 256, 0, s>>>(A, ld, n, b, x);
   ET_CHECK_LAUNCH();
   return ET_OK;

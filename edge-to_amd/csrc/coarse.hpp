@@ -8,11 +8,15 @@ struct CoarseSolver {
   int n = 0;    // matrix order
   int ld = 0;   // padded order (multiple of 64)
   int nbt = 0;  // tiles per side
-  double* A = nullptr;  // after factor(): A^-1 (ld x ld)
-  double* Y = nullptr;  // L^-1 workspace
+  double* A = nullptr;  // after factor(): A^-1 (lib_mode 0, 1) or L^-1 (lib_mode 2), ld x ld
+  double* Y = nullptr;  // lib_mode 0: L^-1 workspace
   int* info = nullptr;    // [0] factor, [1] inverse (device)
   int* info_h = nullptr;  // pinned host copy
-  bool use_lib = true;    // rocSOLVER potrf + potri, else the in-house kernels
+  bool use_lib = true;    // rocBLAS / rocSOLVER handle needed (lib_mode 1, 2)
+  int lib_mode = 2;       // 2: band Cholesky + trtri, 1: rocSOLVER potrf + potri, 0: in-house dense
+  double* Wd = nullptr;   // lib_mode 2: inverse of the current diagonal factor tile (64x64)
+  double* tv = nullptr;   // lib_mode 2: intermediate vector of the coarse solve
+  double* Z = nullptr;    // lib_mode 2: L^-1 row-major (A holds it column-major)
   void* blas = nullptr;   // rocblas_handle
 
   int init(int n, hipStream_t s);
@@ -25,8 +29,10 @@ struct CoarseSolver {
   int solve(const TI* b, TO* x, hipStream_t s) const;
 };
 
-// Factor path of coarse solvers created afterwards: 1 = rocSOLVER (default).
-void set_coarse_lib(int on);
+// Factor path of coarse solvers created afterwards: 2 = band Cholesky (in-house
+// tile kernels, rocBLAS band updates) + rocSOLVER trtri (default), 1 = rocSOLVER
+// potrf + potri, 0 = in-house dense Cholesky and inverse.
+void set_coarse_lib(int mode);
 int get_coarse_lib();
 
 }  // namespace et

@@ -593,6 +593,16 @@ template int level0_op<float>(int, const Grid&, const float*, const float*, cons
 
 // ------------------------------------------------------------ et_mg ----
 
+int MGSolver::coarse_solve(const void* b, void* x) {
+  ET_REQUIRE(built, "et_mg_coarse_solve: no hierarchy yet (call et_mg_solve first)");
+  if (prm.precision == 1)
+    ET_TRY((coarse.solve<float, float>((const float*)b, (float*)x, stream)));
+  else
+    ET_TRY((coarse.solve<double, double>((const double*)b, (double*)x, stream)));
+  ET_CHECK_HIP(hipStreamSynchronize(stream));
+  return ET_OK;
+}
+
 int MGSolver::apply_level(int level, const void* x, void* y) {
   ET_REQUIRE(built, "et_mg_apply_level: no hierarchy yet (call et_mg_solve first)");
   ET_REQUIRE(level >= 0 && level < prm.n_level, "et_mg_apply_level: level must be in [0, n_level)");

@@ -132,6 +132,7 @@ struct MGSolver {
   Prof prof;
   int profile_read(double* ms, int64_t* launches);
   int apply_level(int level, const void* x, void* y);
+  int coarse_solve(const void* b, void* x);
   bool built = false;  // a solve has set up the hierarchy
 
   int create(const et_mg_params& p, const double* K0_host, hipStream_t s);

@@ -42,6 +42,7 @@ _IP = ctypes.POINTER(ctypes.c_int)
 _SIGS = {
     "et_tune_set": [_I, _I],
     "et_mg_apply_level": [_P, _I, _P, _P],
+    "et_mg_coarse_solve": [_P, _P, _P],
     "et_tune_get": [_I],
     "et_generate_elements": [_I, _IP, _P, _P],
     "et_matvec_f64": [_I, _IP, _P, _P, _P, _P, _P, _P],

@@ -59,8 +59,10 @@ const char* et_version(void);
  * ET_TUNE_WHT: 1 (default) = the fine-grid operator (both precisions) runs in the corner
  *   Walsh-Hadamard basis (45 FMAs per element); takes precedence over ET_TUNE_MX_*.
  * ET_TUNE_WHT_WGS: workgroups per compute unit launched by that operator.
- * ET_TUNE_COARSE_LIB: 1 (default) = the coarsest grid is factored and inverted with
- *   rocSOLVER (potrf + potri); 0 = in-house blocked Cholesky kernels.  Read by et_mg_create.
+ * ET_TUNE_COARSE_LIB: factor path of the coarsest grid, read by et_mg_create:
+ *   2 (default) = band Cholesky (in-house tile kernels, rocBLAS band updates) and
+ *   L^-1 by rocSOLVER trtri, coarse solve by two triangular products; 1 = rocSOLVER
+ *   potrf + potri; 0 = in-house dense blocked Cholesky and inverse kernels.
  * ET_TUNE_WHT_THREADS_F32 / _F64: workgroup size (512 or 1024) of the transformed-basis
  *   operator.
  * ET_TUNE_ROW_STENCIL_NODES: coarse levels with at most this many nodes apply their
@@ -255,6 +257,13 @@ int et_mg_solve(et_mg* handle, const double* E, const uint8_t* cons, const doubl
  * 0, float for 1); x and y are device vectors of that level.  No counterpart in the
  * reference: exposes each level's operator for verification.  Synchronises. */
 int et_mg_apply_level(et_mg* handle, int level, const void* x, void* y);
+
+/* x = A_L^-1 b on the coarsest grid L = n_level of the hierarchy built by the
+ * last et_mg_solve, in the V-cycle's precision (double for precision 0, float
+ * for 1).  No counterpart in the reference (its coarsest-grid solve is
+ * scipy's splu inside solvers/_multigrid_cuda.py:120-131): exposes the direct
+ * solve for verification.  Synchronises. */
+int et_mg_coarse_solve(et_mg* handle, const void* b, void* x);
 
 /* Per-phase timings of the last solve in milliseconds (setup, pcg), via HIP events. */
 int et_mg_last_timings(et_mg* handle, double* setup_ms, double* pcg_ms);

@@ -163,3 +163,45 @@ def test_level_operators_match_galerkin(precision, level1_mf, nel):
         ref = A @ x
         err = np.abs(yt.cpu().numpy() - ref).max() / np.abs(ref).max()
         assert err < tol, (level, err)
+
+
+@pytest.mark.parametrize("coarse_lib", [2, 1, 0])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("nel", [(16, 8, 8), (24, 12, 12), (32, 16, 16)])
+def test_coarse_solve_matches_dense(coarse_lib, precision, nel):
+    """The coarsest-grid direct solve (band Cholesky with explicit L^-1, rocSOLVER, in-house
+    dense) against a dense solve of the oracle's coarsest Galerkin operator R A R^T."""
+    ET = _edgeto()
+    from edgeto_amd import _lib as L
+    old = L.lib().et_tune_get(9)
+    L.call("et_tune_set", 9, coarse_lib)
+    n_level = 2
+    try:
+        mesh, kernel, filt, clamp, load = _cantilever_3d(ET, nel)
+        rng = np.random.default_rng(5)
+        E = rng.uniform(0.01, 1.0, mesh.n_el)
+        kernel.set_rho(torch.tensor(E, device="cuda"))
+        tp = ET.CuTopOpt(mesh, ET.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=0.3), kernel,
+                         None, filt, verbose=False)
+        tp.add_BC_nodal(clamp, np.array([[1, 1, 1]]))
+        tp.add_F_nodal(load, np.array([[0.0, -1.0 / len(load), 0.0]]))
+        solver = ET.CuMultiGrid(mesh, kernel, tol=1e-6, n_smooth=2, n_level=n_level, precision=precision)
+        solver.solve(tp.f, use_last=False)
+    finally:
+        L.call("et_tune_set", 9, old)
+    prob = _oracle_problem(nel, 0.3, 2.0, clamp, load, -1.0 / len(load))
+    A = oops.assemble(prob.K0, prob.elements, E, prob.cons, prob.n_dof).tocsr()
+    from oracle import mgm as omgm
+    lnel = tuple(nel)
+    for _ in range(n_level):
+        R = omgm.restriction_matrix(lnel, 3)
+        A = (R @ A @ R.T).tocsr()
+        lnel = tuple(n // 2 for n in lnel)
+    b = rng.standard_normal(A.shape[0])
+    xd = np.linalg.solve(A.toarray(), b)
+    dt = torch.float32 if precision == "mixed" else torch.float64
+    bt = torch.tensor(b, dtype=dt, device="cuda")
+    xt = torch.empty_like(bt)
+    L.call("et_mg_coarse_solve", solver._handle, L.ptr(bt), L.ptr(xt))
+    err = np.abs(xt.cpu().numpy() - xd).max() / np.abs(xd).max()
+    assert err < (1e-5 if precision == "mixed" else 1e-9), err
