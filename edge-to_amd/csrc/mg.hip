@@ -260,13 +260,101 @@ __global__ __launch_bounds__(256) void galerkin_kernel(const Grid gf, const Grid
     for (int q = 0; q < DOF; ++q) Sc[((int64_t)(o * DOF + r) * DOF + q) * Nc + c] = blk[r][q];
 }
 
+// 3-D, dof 3: the same product with the fine pairs (f, g) enumerated per axis.
+// For coarse offset o_a on an axis, f = 2c + d and g = 2t + e (d, e in -1..1)
+// couple through the fine stencil only when s = 2 o_a + e - d is in -1..1:
+// 7 (d, e) pairs for o_a = 0, 3 for o_a = +-1.  Workgroups share o (grid y and
+// the template), so the pair loops are uniform; the z pairs are unrolled and
+// out-of-grid nodes read a clamped node with weight zero, so each (x, y) pair
+// issues all of its loads at once.
+__device__ __forceinline__ int gal_npairs(int oa) { return oa == 0 ? 7 : 3; }
+__device__ __forceinline__ void gal_pair(int oa, int a, int& d, int& e) {
+  // oa = -1: (-1,0) (-1,1) (0,1); oa = 0: |e - d| <= 1; oa = 1: (0,-1) (1,-1) (1,0)
+  if (oa < 0) {
+    d = a == 2 ? 0 : -1;
+    e = a == 0 ? 0 : 1;
+  } else if (oa > 0) {
+    d = a == 0 ? 0 : 1;
+    e = a == 2 ? 0 : -1;
+  } else {
+    constexpr int dd[7] = {-1, -1, 0, 0, 0, 1, 1}, ee[7] = {-1, 0, -1, 0, 1, 0, 1};
+    d = dd[a];
+    e = ee[a];
+  }
+}
+
+template <int OK>
+// two waves per SIMD: lets the scheduler keep all loads of a pair group in flight
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void galerkin3_kernel(const Grid gf, const Grid gc, const double* __restrict__ Sf,
+                                                        double* __restrict__ Sc) {
+  const int64_t Nc = gc.n_nodes(), Nf = gf.n_nodes();
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= Nc) return;
+  const int oi = (int)blockIdx.y / 3 - 1, oj = (int)blockIdx.y % 3 - 1;
+  const int o = (oi + 1) * 9 + (oj + 1) * 3 + (OK + 1);
+  int ci, cj, ck;
+  node_coords<3>(gc, c, ci, cj, ck);
+  const int ti = ci + oi, tj = cj + oj, tk = ck + OK;
+  const int fnx = gf.nx(), fny = gf.ny(), fnz = gf.nz();
+  double blk[3][3] = {};
+  const bool tin = ti >= 0 && ti < gc.nx() && tj >= 0 && tj < gc.ny() && tk >= 0 && tk < gc.nz();
+  if (tin) {
+    const int nxp = gal_npairs(oi), nyp = gal_npairs(oj);
+    constexpr int nzp = OK == 0 ? 7 : 3;
+    for (int a = 0; a < nxp; ++a) {
+      int di, ei;
+      gal_pair(oi, a, di, ei);
+      const int fi = 2 * ci + di, gi = 2 * ti + ei;
+      const bool xin = fi >= 0 && fi < fnx && gi >= 0 && gi < fnx;
+      const int si = 2 * oi + ei - di;
+      for (int b2 = 0; b2 < nyp; ++b2) {
+        int dj, ej;
+        gal_pair(oj, b2, dj, ej);
+        const int fj = 2 * cj + dj, gj = 2 * tj + ej;
+        const bool yin = fj >= 0 && fj < fny && gj >= 0 && gj < fny;
+        const int sj = 2 * oj + ej - dj;
+        const double wxy = (xin && yin) ? pw(di) * pw(ei) * pw(dj) * pw(ej) : 0.0;
+        const int fic = min(max(fi, 0), fnx - 1), fjc = min(max(fj, 0), fny - 1);
+#pragma unroll
+        for (int q3 = 0; q3 < nzp; ++q3) {
+          int dk, ek;
+          gal_pair(OK, q3, dk, ek);
+          const int fk = 2 * ck + dk, gk = 2 * tk + ek;
+          const bool zin = fk >= 0 && fk < fnz && gk >= 0 && gk < fnz;
+          const double w = zin ? wxy * pw(dk) * pw(ek) : 0.0;
+          const int sk = 2 * OK + ek - dk;
+          const int64_t f = ((int64_t)fjc * fnx + fic) * fnz + min(max(fk, 0), fnz - 1);
+          const int so = (si + 1) * 9 + (sj + 1) * 3 + (sk + 1);
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) blk[r][q] += w * Sf[((int64_t)(so * 3 + r) * 3 + q) * Nf + f];
+        }
+      }
+    }
+    const double inv = 1.0 / (tw_of<3>(gc, ci, cj, ck) * tw_of<3>(gc, ti, tj, tk));
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) blk[r][q] *= inv;
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) Sc[((int64_t)(o * 3 + r) * 3 + q) * Nc + c] = blk[r][q];
+}
+
 int galerkin(const Grid& gf, int dof, const double* Sf, double* Sc, hipStream_t s) {
   const Grid gc = gf.coarse();
   const int NO = gf.dim == 3 ? 27 : 9;
   const unsigned grid = grid_for(gc.n_nodes() * NO);
   if (gf.dim == 3) {
     ET_REQUIRE(dof == 3, "3-D Galerkin expects dof 3");
-    galerkin_kernel<3, 3><<<grid, kBlock, 0, s>>>(gf, gc, Sf, Sc);
+    const dim3 g3(grid_for(gc.n_nodes()), 9);
+    galerkin3_kernel<-1><<<g3, kBlock, 0, s>>>(gf, gc, Sf, Sc);
+    galerkin3_kernel<0><<<g3, kBlock, 0, s>>>(gf, gc, Sf, Sc);
+    galerkin3_kernel<1><<<g3, kBlock, 0, s>>>(gf, gc, Sf, Sc);
+    (void)grid;
   } else {
     ET_REQUIRE(dof == 2, "2-D Galerkin expects dof 2");
     galerkin_kernel<2, 2><<<grid, kBlock, 0, s>>>(gf, gc, Sf, Sc);
