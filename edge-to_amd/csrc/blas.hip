@@ -38,6 +38,12 @@ int reduce_finalize_n(const double* partial, int m, double* out, hipStream_t s) 
   return ET_OK;
 }
 
+int dot_partials(int64_t n, const double* a, const double* b, double* partial, hipStream_t s) {
+  dot_partial_kernel<<<kRedBlocks, 256, 0, s>>>(n, a, b, partial);
+  ET_CHECK_LAUNCH();
+  return ET_OK;
+}
+
 int dot(int64_t n, const double* a, const double* b, double* partial, double* out, hipStream_t s) {
   dot_partial_kernel<<<kRedBlocks, 256, 0, s>>>(n, a, b, partial);
   ET_CHECK_LAUNCH();

@@ -308,9 +308,17 @@ __global__ __launch_bounds__(256) void trmv_lower_kernel(const double* __restric
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
   const double* z = Z + (int64_t)i * ld;
-  double s = 0.0;
-  for (int j = lane; j <= i; j += 64) s += z[j] * (double)b[j];
-  s = wave_sum(s);
+  // four independent partial sums: four loads of each operand in flight per lane
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int j = lane;
+  for (; j + 192 <= i; j += 256) {
+    s0 += z[j] * (double)b[j];
+    s1 += z[j + 64] * (double)b[j + 64];
+    s2 += z[j + 128] * (double)b[j + 128];
+    s3 += z[j + 192] * (double)b[j + 192];
+  }
+  for (; j <= i; j += 64) s0 += z[j] * (double)b[j];
+  const double s = wave_sum((s0 + s1) + (s2 + s3));
   if (lane == 0) t[i] = s;
 }
 
@@ -322,9 +330,16 @@ __global__ __launch_bounds__(256) void trmv_upper_kernel(const double* __restric
   const int lane = threadIdx.x & 63;
   if (j >= n) return;
   const double* y = Y + (int64_t)j * ld;
-  double s = 0.0;
-  for (int i = j + lane; i < n; i += 64) s += y[i] * t[i];
-  s = wave_sum(s);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int i = j + lane;
+  for (; i + 192 < n; i += 256) {
+    s0 += y[i] * t[i];
+    s1 += y[i + 64] * t[i + 64];
+    s2 += y[i + 128] * t[i + 128];
+    s3 += y[i + 192] * t[i + 192];
+  }
+  for (; i < n; i += 64) s0 += y[i] * t[i];
+  const double s = wave_sum((s0 + s1) + (s2 + s3));
   if (lane == 0) x[j] = (TO)s;
 }
 

@@ -110,4 +110,14 @@ __device__ inline T block_sum256(T v) {
   return r;
 }
 
+// block_sum256 with the total returned to every thread of the block
+template <typename T>
+__device__ inline T block_sum256_all(T v) {
+  __shared__ T tot;
+  v = block_sum256(v);
+  if (threadIdx.x == 0) tot = v;
+  __syncthreads();
+  return tot;
+}
+
 }  // namespace et

@@ -279,7 +279,8 @@ __global__ void stencil_dinv_kernel(int64_t N, const double* __restrict__ S, T* 
 // precision copy of r for the V-cycle; per-block partial sums of r.r.  A no-op
 // once the loop has stopped (SC_STOP).
 template <typename TB>
-__global__ __launch_bounds__(256) void pcg_update_xr_kernel(int64_t n, const double* __restrict__ sc,
+__global__ __launch_bounds__(256) void pcg_update_xr_kernel(int64_t n, double* __restrict__ sc, int cur,
+                                                            const double* __restrict__ pq_part, int n_pq,
                                                             const double* __restrict__ p,
                                                             const double* __restrict__ q,
                                                             double* __restrict__ x,
@@ -290,7 +291,12 @@ __global__ __launch_bounds__(256) void pcg_update_xr_kernel(int64_t n, const dou
     if (threadIdx.x == 0) partial[blockIdx.x] = 0.0;
     return;
   }
-  const double alpha = sc[SC_RHO] / sc[SC_PQ];
+  // p.q from its partials: every block sums them in the same fixed order
+  double pq = 0.0;
+  for (int t = threadIdx.x; t < n_pq; t += blockDim.x) pq += pq_part[t];
+  pq = block_sum256_all(pq);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[SC_PQ] = pq;
+  const double alpha = sc[SC_RHO + cur] / pq;
   double s = 0.0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -317,12 +323,19 @@ __global__ __launch_bounds__(256) void pcg_dot_kernel(int64_t n, const double* _
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// p = z + beta p with beta = rho_new / rho (Fletcher-Reeves); p = z when first
+// p = z + beta p, beta = rho_new / rho (Fletcher-Reeves); p = z when first.
+// rho_new = r.z is summed from its partials by every block in the same fixed
+// order and stored in slot SC_RHO + nxt; the previous rho is in the other slot.
 template <typename TZ>
-__global__ __launch_bounds__(256) void pcg_update_p_kernel(int64_t n, const double* __restrict__ sc,
+__global__ __launch_bounds__(256) void pcg_update_p_kernel(int64_t n, double* __restrict__ sc, int nxt,
+                                                           const double* __restrict__ rz_part, int n_rz,
                                                            const TZ* __restrict__ z,
                                                            double* __restrict__ p, int first) {
-  const double beta = first ? 0.0 : sc[SC_RHO_NEW] / sc[SC_RHO];
+  double rz = 0.0;
+  for (int t = threadIdx.x; t < n_rz; t += blockDim.x) rz += rz_part[t];
+  rz = block_sum256_all(rz);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[SC_RHO + nxt] = rz;
+  const double beta = first ? 0.0 : rz / sc[SC_RHO + (nxt ^ 1)];
   // four consecutive values per step (z and p are 16-byte aligned allocations)
   const int64_t n4 = n >> 2;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4;
@@ -343,15 +356,21 @@ __global__ __launch_bounds__(256) void pcg_update_p_kernel(int64_t n, const doub
     p[t] = first ? (double)z[t] : (double)z[t] + beta * p[t];
 }
 
-__global__ void pcg_rotate_kernel(double* sc) { sc[SC_RHO] = sc[SC_RHO_NEW]; }
-
 // Convergence / breakdown test of one iteration, evaluated on the device so the
-// host never waits on the current iteration: R = |r| / |b|; non-finite p.q or
-// r.r, p.q <= 0 and R < tol stop the loop (first reason wins, as in a plain
-// sequential loop).  {R, status, stop} is published to pinned host memory.
-__global__ void pcg_check_kernel(double* sc, double tol, double* ring) {
+// host never waits on the current iteration: r.r from the update's partials,
+// R = |r| / |b|; non-finite p.q or r.r, p.q <= 0 and R < tol stop the loop
+// (first reason wins, as in a plain sequential loop).  While running, omega =
+// min + d/2 exp((-clip(R, tol, 0.1) + tol) * 500) for the next V-cycle.
+// {R, status, stop} is published to pinned host memory.  One block of 256.
+__global__ __launch_bounds__(256) void pcg_check_kernel(double* sc, const double* __restrict__ rr_part, int n_rr,
+                                                        double tol, double wmin, double dw, double* ring) {
+  double rr = 0.0;
+  for (int t = threadIdx.x; t < n_rr; t += blockDim.x) rr += rr_part[t];
+  rr = block_sum256(rr);
+  if (threadIdx.x != 0) return;
   if (sc[SC_STOP] == 0.0) {
-    const double pq = sc[SC_PQ], rr = sc[SC_RR];
+    sc[SC_RR] = rr;
+    const double pq = sc[SC_PQ];
     const double R = sqrt(rr) / sqrt(sc[SC_BB]);
     double status = (double)ET_OK, stop = 0.0;
     if (!isfinite(pq) || !isfinite(rr)) {
@@ -366,19 +385,15 @@ __global__ void pcg_check_kernel(double* sc, double tol, double* ring) {
     sc[SC_R] = R;
     sc[SC_STATUS] = status;
     sc[SC_STOP] = stop;
+    if (stop == 0.0) {
+      const double Rc = fmin(fmax(R, tol), 1e-1);
+      sc[SC_OMEGA] = wmin + dw / 2.0 * exp((-Rc + tol) * 500.0);
+    }
   }
   ring[0] = sc[SC_R];
   ring[1] = sc[SC_STATUS];
   ring[2] = sc[SC_STOP];
   __threadfence_system();
-}
-
-// omega = min + d/2 exp((-clip(R, tol, 0.1) + tol) * 500), R = |r| / |b|
-__global__ void pcg_omega_kernel(double* sc, double tol, double wmin, double dw) {
-  if (sc[SC_STOP] != 0.0) return;
-  const double R = sqrt(sc[SC_RR]) / sqrt(sc[SC_BB]);
-  const double Rc = fmin(fmax(R, tol), 1e-1);
-  sc[SC_OMEGA] = wmin + dw / 2.0 * exp((-Rc + tol) * 500.0);
 }
 
 __global__ void pcg_init_kernel(double* sc, double omega) {
@@ -666,7 +681,7 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
     ET_CHECK_HIP(hipStreamSynchronize(s));
   }
   ET_CHECK_HIP(hipMalloc(&sh.scal, SC_COUNT * sizeof(double)));
-  ET_CHECK_HIP(hipMalloc(&partial, kRedBlocks * sizeof(double)));
+  ET_CHECK_HIP(hipMalloc(&partial, 2 * kRedBlocks * sizeof(double)));
   ET_CHECK_HIP(hipHostMalloc(&ring, 8 * sizeof(double), hipHostMallocCoherent));
   ET_CHECK_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&ring_dev), ring, 0));
   for (auto& e : ring_ev) ET_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -877,15 +892,16 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
   const T* z = nullptr;
   ET_TRY(precondition(h, false, &z));
   prof.begin(ET_PROF_PCG_VEC, s);
+  // partial: first-stage sums of p.q and r.z; partial2: of r.r
+  double* partial2 = partial + kRedBlocks;
   pcg_dot_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, r, z, partial);
-  ET_TRY(reduce_finalize(partial, sh.scal + SC_RHO, s));
-  pcg_update_p_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, z, p_, 1);
+  pcg_update_p_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, 0, partial, kRedBlocks, z, p_, 1);
   prof.end(s);
   ET_CHECK_LAUNCH();
   // the fused q = A p, p.q path exists for the transformed-basis operator
   const bool fused_pq = sh.tiled && g0.dim == 3 && get_wht() && wht_supported(g0, sizeof(double)) &&
                         wht_ydot_partials(g0) <= kRedBlocks;
-  const int n_pq = fused_pq ? wht_ydot_partials(g0) : 0;
+  const int n_pq_fused = fused_pq ? wht_ydot_partials(g0) : 0;
   int last = 0;
   auto read_ring = [&](int k, double* R, int* st) -> bool {
     ET_CHECK_HIP(hipEventSynchronize(ring_ev[k & 1]));
@@ -898,23 +914,27 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
   int status = ET_OK;
   bool stopped = false;
   for (int it = 1; it <= maxiter; ++it) {
+    // rho of this iteration is in slot SC_RHO + cur; the new one goes to the other
+    const int cur = (it - 1) & 1;
+    int n_pq = kRedBlocks;
     if (fused_pq) {
       // q = A p with p.q summed per workgroup inside the operator
       prof.begin(ET_PROF_MATVEC64, s);
       ET_TRY(op0_wht_ydot<double>(g0, (int64_t)g0.ey * g0.ez, g0.ez, sh.K0d, sh.Ed, sh.cons, p_, q, partial,
                                   kRedBlocks, s));
       prof.end(s);
+      n_pq = n_pq_fused;
       prof.begin(ET_PROF_PCG_VEC, s);
-      ET_TRY(reduce_finalize_n(partial, n_pq, sh.scal + SC_PQ, s));
     } else {
       ET_TRY(apply_A64(p_, q, OP_Y, nullptr));
       prof.begin(ET_PROF_PCG_VEC, s);
-      ET_TRY(dot(n_dof, p_, q, partial, sh.scal + SC_PQ, s));
+      ET_TRY(dot_partials(n_dof, p_, q, partial, s));
     }
-    pcg_update_xr_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, p_, q, x, r, rb, partial);
+    pcg_update_xr_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, cur, partial, n_pq, p_, q, x, r, rb,
+                                                       partial2);
     ET_CHECK_LAUNCH();
-    ET_TRY(reduce_finalize(partial, sh.scal + SC_RR, s));
-    pcg_check_kernel<<<1, 1, 0, s>>>(sh.scal, tol, ring_dev + 4 * (it & 1));
+    pcg_check_kernel<<<1, 256, 0, s>>>(sh.scal, partial2, kRedBlocks, tol, min_omega, d_omega,
+                                       ring_dev + 4 * (it & 1));
     ET_CHECK_LAUNCH();
     ET_CHECK_HIP(hipEventRecord(ring_ev[it & 1], s));
     prof.end(s);
@@ -924,14 +944,10 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
       stopped = true;
       break;
     }
-    pcg_omega_kernel<<<1, 1, 0, s>>>(sh.scal, tol, min_omega, d_omega);
-    ET_CHECK_LAUNCH();
     ET_TRY(precondition(h, true, &z));
     prof.begin(ET_PROF_PCG_VEC, s);
     pcg_dot_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, r, z, partial);
-    ET_TRY(reduce_finalize(partial, sh.scal + SC_RHO_NEW, s));
-    pcg_update_p_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, z, p_, 0);
-    pcg_rotate_kernel<<<1, 1, 0, s>>>(sh.scal);
+    pcg_update_p_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, it & 1, partial, kRedBlocks, z, p_, 0);
     prof.end(s);
     ET_CHECK_LAUNCH();
   }

@@ -9,7 +9,8 @@
 namespace et {
 
 // device scalar slots of the PCG loop
-enum { SC_BB = 0, SC_RHO, SC_RHO_NEW, SC_PQ, SC_RR, SC_OMEGA, SC_R, SC_STATUS, SC_STOP, SC_COUNT };
+// SC_RHO, SC_RHO + 1: r.z of consecutive PCG iterations (slots alternate)
+enum { SC_BB = 0, SC_RHO, SC_RHO_ALT, SC_PQ, SC_RR, SC_OMEGA, SC_R, SC_STATUS, SC_STOP, SC_COUNT };
 
 // Event-based per-kernel-class timing (enabled on demand).
 struct Prof {
@@ -122,7 +123,7 @@ struct MGSolver {
   bool use_l1 = false;
 
   double *r = nullptr, *z = nullptr, *p_ = nullptr, *q = nullptr, *diag64 = nullptr;
-  double* partial = nullptr;
+  double* partial = nullptr;  // 2 x kRedBlocks first-stage reduction sums
   double* ring = nullptr;      // pinned host slots {R, status, stop} of the last two iterations
   double* ring_dev = nullptr;  // device address of ring
   hipEvent_t ring_ev[2] = {nullptr, nullptr};
