@@ -48,7 +48,6 @@ namespace et {
 // values, (TC-1)*nz <= 1024 output nodes.
 constexpr int kWStageCap = 1536, kWECap = 1024, kWNodeCap = 1024;
 constexpr int kWG = 63;  // transformed blocks M = 1..7, 3 x 3 each
-__device__ const uint8_t kWZeroByte[4] = {0, 0, 0, 0};
 
 // workgroup of NT threads (512 or 1024)
 template <int NT>
@@ -143,9 +142,13 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
   const int wave_u = __builtin_amdgcn_readfirstlane(wave);
   // 32-bit DOF offsets: the host checks 3 * n_nodes < 2^31
   const int plane3 = nx * nz * 3;
-  // without constraints every constraint-byte read goes to a zero byte
-  const uint8_t* cp = cons != nullptr ? cons : kWZeroByte;
+  // Without constraints every constraint-byte read goes to byte 0 of E and is
+  // masked to zero.  Both candidates are kernel arguments, so the loads stay
+  // global loads (a pointer that may also point at a device-side constant
+  // becomes a flat load, which also counts against the LDS wait counter).
+  const uint8_t* cp = cons != nullptr ? cons : reinterpret_cast<const uint8_t*>(E);
   const int cmask = cons != nullptr ? ~0 : 0;
+  const uint8_t cbyte = cons != nullptr ? (uint8_t)0xff : (uint8_t)0;
   constexpr int kWThreads = NT;
   constexpr int kWSlots = WCfg<NT>::slots;
   constexpr int kWStage = WCfg<NT>::stage;
@@ -169,15 +172,19 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
     sg[tid] = (T)(s / 64.0);
   }
   __syncthreads();
-  // the 45 non-zero coefficients are wave-uniform: scalar registers
+  // The 45 non-zero coefficients are wave-uniform: scalar registers.  G is
+  // symmetric, G[M][c][d] = G[M ^ e_c ^ e_d][d][c], so only the 33 entries with
+  // c <= d are kept (fewer scalar registers to spill to vector lanes).
   T gk[kWG];
 #pragma unroll
   for (int t = 0; t < kWG; ++t) {
     const int M = 1 + t / 9, c = (t / 3) % 3, d = t % 3;
     gk[t] = T(0);
-    if (wht_nz(M, c, d)) gk[t] = wht_uniform(sg[t]);
+    if (wht_nz(M, c, d) && c <= d) gk[t] = wht_uniform(sg[t]);
   }
-  auto gval = [&](int t) -> T { return gk[t]; };
+  auto gval = [&](int M, int c, int d) -> T {
+    return c <= d ? gk[(M - 1) * 9 + c * 3 + d] : gk[((M ^ (1 << c) ^ (1 << d)) - 1) * 9 + d * 3 + c];
+  };
   const T w = kJac ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
   double dot = 0.0;  // TM_YDOT: this thread's share of x.y
 
@@ -250,7 +257,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
             } else {
               pu[m][r] = x[d + r];
             }
-            pk[m][r] = cp[(d + r) & cmask];
+            pk[m][r] = cp[(d + r) & cmask] & cbyte;
           }
         }
       }
@@ -338,7 +345,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
         dd[q] = act[q] ? jb3 + osl[q] * 3 : 0;
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-          cq[q][r] = cp[(dd[q] + r) & cmask];
+          cq[q][r] = cp[(dd[q] + r) & cmask] & cbyte;
           bq[q][r] = !kProd ? b[dd[q] + r] : T(0);
           dq[q][r] = kJac ? Dinv[dd[q] + r] : T(0);
           aq[q][r] = MODE == TM_JAC_ADD ? xadd[dd[q] + r] : T(0);
@@ -373,7 +380,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
             T s = T(0);
 #pragma unroll
             for (int d = 0; d < 3; ++d)
-              if (wht_nz(M, c, d)) s = fma(gval((M - 1) * 9 + c * 3 + d), uh[d][M ^ (1 << c) ^ (1 << d)], s);
+              if (wht_nz(M, c, d)) s = fma(gval(M, c, d), uh[d][M ^ (1 << c) ^ (1 << d)], s);
             yh[c][M] = s;
           }
         }
