@@ -208,7 +208,7 @@ def measured_traffic(kernel_class, workload):
             os.path.relpath(path, ROOT))
 
 
-ROOFLINE_CLASSES = ("l0_jacobi", "l0_residual", "l0_jacobi_add", "matvec64")
+ROOFLINE_CLASSES = ("l0_jacobi", "l0_jacobi0", "l0_residual", "l0_jacobi_add", "matvec64")
 
 
 def roofline(prof, n_dof, n_el, workload):
@@ -217,6 +217,7 @@ def roofline(prof, n_dof, n_el, workload):
     mask = n_dof  # one byte per DOF
     bytes_per = {  # algorithmic bytes per launch
         "l0_jacobi": s * (4 * n_dof + n_el) + mask,       # read x, b, D^-1, E; write y
+        "l0_jacobi0": s * (3 * n_dof + n_el) + mask,      # read b, D^-1, E; write y (x0 = w D^-1 b)
         "l0_residual": s * (3 * n_dof + n_el) + mask,     # read x, b, E; write r
         "l0_jacobi_add": s * (5 * n_dof + n_el) + mask,   # + read x_prev
         "matvec64": 8 * (2 * n_dof + n_el) + mask,        # FP64: read x, E; write y

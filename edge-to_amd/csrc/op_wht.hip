@@ -346,7 +346,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
           cq[q][r] = cp[(dd[q] + r) & cmask] & cbyte;
-          bq[q][r] = !kProd ? b[dd[q] + r] : T(0);
+          bq[q][r] = (!kProd && !kZero) ? b[dd[q] + r] : T(0);
           dq[q][r] = kJac ? Dinv[dd[q] + r] : T(0);
           aq[q][r] = MODE == TM_JAC_ADD ? xadd[dd[q] + r] : T(0);
         }
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
       }
       // the node's own (staged, constrained-zeroed) value is read before the
       // barrier: after it, the slot of plane ey is refilled by other threads
-      if (MODE != TM_Y && !kZero) {
+      if (MODE != TM_Y) {
         const T* own = su + (size_t)(ey & 1) * 3 * span;
 #pragma unroll
         for (int q = 0; q < kWNodes; ++q) {
@@ -443,21 +443,28 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
               if (seam)
                 ax0 += ss[((r * 2) * tc + nl) * 2 + (k >> 6) - 1] + ss[((r * 2 + 1) * tc + nl - 1) * 2 + (k >> 6) - 1];
               const bool cr = cq[q][r] != 0;
-              T xv;
-              if (kZero) {
-                xv = w * dq[q][r] * bq[q][r];  // x0 of this DOF (constrained or not)
-              } else {
-                xv = xq[q][r];  // zero on constrained DOFs
-                if (cr) xv = x[d];
-              }
-              const T ax = cr ? xv : ax0;
+              T xv = xq[q][r];  // staged value: zero on constrained DOFs
               T vv;
-              if (kProd)
-                vv = ax;
-              else if (MODE == TM_RES)
-                vv = bq[q][r] - ax;
-              else
-                vv = xv + w * dq[q][r] * (bq[q][r] - ax);
+              if (kZero) {
+                // x0 = w D^-1 b is the staged value, so x0 + w D^-1 (b - A x0) = 2 x0 - w D^-1 A x0
+                // (b itself is read only on constrained DOFs, where A x0 = x0)
+                if (cr) {
+                  const T bv = b[d];
+                  xv = w * dq[q][r] * bv;
+                  vv = xv + w * dq[q][r] * (bv - xv);
+                } else {
+                  vv = xv + xv - w * dq[q][r] * ax0;
+                }
+              } else {
+                if (cr) xv = x[d];
+                const T ax = cr ? xv : ax0;
+                if (kProd)
+                  vv = ax;
+                else if (MODE == TM_RES)
+                  vv = bq[q][r] - ax;
+                else
+                  vv = xv + w * dq[q][r] * (bq[q][r] - ax);
+              }
               if (MODE == TM_JAC_ADD) vv += aq[q][r];
               if (MODE == TM_YDOT) dot += (double)xv * (double)vv;
               y[d] = vv;

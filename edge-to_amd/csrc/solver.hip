@@ -409,6 +409,7 @@ __global__ void set_scalar_kernel(double* sc, int slot, double v) { sc[slot] = v
 
 static int op_kind(int level, int mode) {
   if (level > 0) return ET_PROF_COARSE_OPS;
+  if (mode == OP_JAC0) return ET_PROF_L0_JAC0;
   return mode == OP_RES ? ET_PROF_L0_RES : (mode == OP_JAC_ADD ? ET_PROF_L0_JACADD : ET_PROF_L0_JAC);
 }
 

@@ -78,7 +78,7 @@ _SIGS = {
 }
 
 PROF_KINDS = ["l0_jacobi", "l0_residual", "l0_jacobi_add", "l0_scale", "matvec64", "coarse_ops",
-              "transfer", "coarse_solve", "pcg_vector", "setup"]
+              "transfer", "coarse_solve", "pcg_vector", "setup", "l0_jacobi0"]
 
 EXPORTED = sorted(list(_SIGS) + ["et_last_error", "et_version"])
 

@@ -286,7 +286,8 @@ enum {
   ET_PROF_COARSE_SOLVE = 7,/* dense coarsest-grid solve                               */
   ET_PROF_PCG_VEC = 8,     /* PCG dots, updates and precision conversions             */
   ET_PROF_SETUP = 9,       /* hierarchy construction and coarse factorisation         */
-  ET_PROF_KINDS = 10
+  ET_PROF_L0_JAC0 = 10,    /* first fine-grid sweep fused with its zero-guess scaling  */
+  ET_PROF_KINDS = 11
 };
 int et_mg_profile(et_mg* handle, int enable);
 int et_mg_profile_read(et_mg* handle, double* ms, int64_t* launches);
