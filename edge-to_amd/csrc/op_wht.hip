@@ -371,48 +371,42 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
             uh[c][M0 | 2] = e * (Fbm[c][f] - Ft[c][f]);
             Fbm[c][f] = Ft[c][f];
           }
-        T yh[3][8];
+        // one output component at a time: its 8 transformed values, the y stage
+        // and face transform of the inverse, and its node contributions (so
+        // only 8 of the 24 transformed outputs are live at once)
+        const int k = ez;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          yh[c][0] = T(0);
+          T yc[8];
+          yc[0] = T(0);
 #pragma unroll
           for (int M = 1; M < 8; ++M) {
             T s = T(0);
 #pragma unroll
             for (int d = 0; d < 3; ++d)
               if (wht_nz(M, c, d)) s = fma(gval(M, c, d), uh[d][M ^ (1 << c) ^ (1 << d)], s);
-            yh[c][M] = s;
+            yc[M] = s;
           }
-        }
-        // y stage of the inverse; this layer's bottom face completes node plane ey
-        T v[3][2][2];  // [c][hx][hz]
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
+          // y stage of the inverse; this layer's bottom face completes node plane ey
           T Yb[4];
 #pragma unroll
           for (int f = 0; f < 4; ++f) {
             const int M0 = (f & 1) | ((f >> 1) << 2);
-            Yb[f] = yh[c][M0] + yh[c][M0 | 2] + Yc[m][c][f];
-            Yc[m][c][f] = yh[c][M0] - yh[c][M0 | 2];
+            Yb[f] = yc[M0] + yc[M0 | 2] + Yc[m][c][f];
+            Yc[m][c][f] = yc[M0] - yc[M0 | 2];
           }
-          T o[4];
+          T o[4];  // [hx + 2 hz]
           wht_face(Yb[0], Yb[1], Yb[2], Yb[3], o);  // symmetric; the normalisation is in G
-          v[c][0][0] = o[0];
-          v[c][1][0] = o[1];
-          v[c][0][1] = o[2];
-          v[c][1][1] = o[3];
-        }
-        // z: the hz = 1 contributions move one lane up; lane 63's go to the seam slot
-        const int k = ez;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
+          // z: the hz = 1 contributions move one lane up; lane 63's go to the seam slot
 #pragma unroll
           for (int hx = 0; hx < 2; ++hx) {
-            const T up = v[c][hx][1];
-            const T zv = v[c][hx][0] + wht_shr(up);
+            const T up = o[hx + 2];
+            const T zv = o[hx] + wht_shr(up);
             if (out && k <= nelz) sw[((c * 2 + hx) * tc + cl) * nz + k] = zv;
             if (out && lane == 63) ss[((c * 2 + hx) * tc + cl) * 2 + (ez >> 6)] = up;
           }
+          if (c < 2) __builtin_amdgcn_sched_barrier(0);
+        }
       }
       // the node's own (staged, constrained-zeroed) value is read before the
       // barrier: after it, the slot of plane ey is refilled by other threads
