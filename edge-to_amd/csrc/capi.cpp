@@ -89,6 +89,8 @@ int et_tune_set(int param, int value) {
       et::set_wht_threads(param == ET_TUNE_WHT_THREADS_F64, value);
     } else if (param == ET_TUNE_LEVEL1_MF) {
       et::set_level1_mf(value);
+    } else if (param == ET_TUNE_FUSED_RZ) {
+      et::set_fused_rz(value);
     } else if (param == ET_TUNE_COARSE_LIB) {
       ET_REQUIRE(value >= 0 && value <= 2, "et_tune_set: coarse factor path must be 0, 1 or 2");
       et::set_coarse_lib(value);
@@ -114,6 +116,7 @@ int et_tune_get(int param) {
   if (param == ET_TUNE_WHT) return et::get_wht();
   if (param == ET_TUNE_COARSE_LIB) return et::get_coarse_lib();
   if (param == ET_TUNE_LEVEL1_MF) return et::get_level1_mf();
+  if (param == ET_TUNE_FUSED_RZ) return et::get_fused_rz();
   if (param == ET_TUNE_ROW_STENCIL_NODES) return (int)et::get_row_stencil_nodes();
   if (param == ET_TUNE_WHT_THREADS_F32 || param == ET_TUNE_WHT_THREADS_F64)
     return et::get_wht_threads(param == ET_TUNE_WHT_THREADS_F64);

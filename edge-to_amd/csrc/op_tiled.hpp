@@ -40,7 +40,7 @@ constexpr int kESlots = 4;  // E values staged per thread per plane (e_span <= 1
 // Transformed-basis operator only: TM_YDOT, y = A x plus per-workgroup partial
 // sums of x.y; TM_JAC0, the Jacobi sweep from the zero guess fused with its
 // first step, y = x0 + w D^-1 (b - A x0) with x0 = w D^-1 b (x is not read).
-enum TiledMode { TM_Y = 0, TM_RES = 1, TM_JAC = 2, TM_JAC_ADD = 3, TM_YDOT = 4, TM_JAC0 = 5 };
+enum TiledMode { TM_Y = 0, TM_RES = 1, TM_JAC = 2, TM_JAC_ADD = 3, TM_YDOT = 4, TM_JAC0 = 5, TM_JAC_ADD_DOT = 6 };
 
 struct TiledArgs {
   Grid g;

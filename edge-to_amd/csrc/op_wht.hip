@@ -122,9 +122,12 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
                                                             const T* __restrict__ Dinv,
                                                             const double* __restrict__ omega,
                                                             double wscale, const T* __restrict__ xadd,
-                                                            T* __restrict__ y, double* __restrict__ partial) {
+                                                            T* __restrict__ y, double* __restrict__ partial,
+                                                            const T* __restrict__ dotv) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  constexpr bool kJac = MODE == TM_JAC || MODE == TM_JAC_ADD || MODE == TM_JAC0;
+  constexpr bool kAdd = MODE == TM_JAC_ADD || MODE == TM_JAC_ADD_DOT;  // output + xadd
+  constexpr bool kJac = MODE == TM_JAC || MODE == TM_JAC0 || kAdd;
+  constexpr bool kDot = MODE == TM_YDOT || MODE == TM_JAC_ADD_DOT;  // per-workgroup partial sums
   constexpr bool kZero = MODE == TM_JAC0;  // input x0 = w D^-1 b, built while staging
   constexpr bool kProd = MODE == TM_Y || MODE == TM_YDOT;  // output is A x
   const Grid& g = A.g;
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
     return c <= d ? gk[(M - 1) * 9 + c * 3 + d] : gk[((M ^ (1 << c) ^ (1 << d)) - 1) * 9 + d * 3 + c];
   };
   const T w = kJac ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
-  double dot = 0.0;  // TM_YDOT: this thread's share of x.y
+  double dot = 0.0;  // this thread's share of x.y (TM_YDOT) or dotv.y (TM_JAC_ADD_DOT)
 
   // element slots: slice s = wave + 8 m -> column s / spc, z range 64 (s % spc) + [0, 64)
   int scl[kWSlots], sez[kWSlots];
@@ -338,6 +341,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
       bool act[kWNodes];
       int dd[kWNodes];
       T bq[kWNodes][3], dq[kWNodes][3], aq[kWNodes][3], xq[kWNodes][3];
+      T vq[MODE == TM_JAC_ADD_DOT ? kWNodes : 1][3];
       uint8_t cq[kWNodes][3];
 #pragma unroll
       for (int q = 0; q < kWNodes; ++q) {
@@ -348,7 +352,8 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
           cq[q][r] = cp[(dd[q] + r) & cmask] & cbyte;
           bq[q][r] = (!kProd && !kZero) ? b[dd[q] + r] : T(0);
           dq[q][r] = kJac ? Dinv[dd[q] + r] : T(0);
-          aq[q][r] = MODE == TM_JAC_ADD ? xadd[dd[q] + r] : T(0);
+          aq[q][r] = kAdd ? xadd[dd[q] + r] : T(0);
+          if (MODE == TM_JAC_ADD_DOT) vq[MODE == TM_JAC_ADD_DOT ? q : 0][r] = dotv[dd[q] + r];
         }
       }
 
@@ -459,8 +464,9 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
                 else
                   vv = xv + w * dq[q][r] * (bq[q][r] - ax);
               }
-              if (MODE == TM_JAC_ADD) vv += aq[q][r];
+              if (kAdd) vv += aq[q][r];
               if (MODE == TM_YDOT) dot += (double)xv * (double)vv;
+              if (MODE == TM_JAC_ADD_DOT) dot += (double)vq[MODE == TM_JAC_ADD_DOT ? q : 0][r] * (double)vv;
               y[d] = vv;
             }
           }
@@ -473,7 +479,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
       __syncthreads();
     }
   }
-  if (MODE == TM_YDOT) {
+  if (kDot) {
     // fixed-order workgroup sum: waves, then wave partials in index order; the
     // wave partials reuse the (now idle) dynamic LDS, so the kernel's static LDS
     // stays zero and the full dynamic allowance remains available
@@ -526,24 +532,26 @@ static int wht_device_cus() {
 template <typename T, int MODE, int NT>
 static void launch_wht_nt(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const T* K0, const T* E,
                           const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
-                          double wscale, const T* xadd, T* y, double* partial) {
+                          double wscale, const T* xadd, T* y, double* partial, const T* dotv) {
   static bool attr_set = false;
   if (!attr_set) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&op0_wht_kernel<T, MODE, NT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWMaxLds);
     attr_set = true;
   }
-  op0_wht_kernel<T, MODE, NT><<<n_wg, NT, lds, s>>>(A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial);
+  op0_wht_kernel<T, MODE, NT><<<n_wg, NT, lds, s>>>(A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial,
+                                                    dotv);
 }
 
 template <typename T, int MODE>
 static void launch_wht(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const T* K0, const T* E,
                        const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
-                       double wscale, const T* xadd, T* y, double* partial = nullptr) {
+                       double wscale, const T* xadd, T* y, double* partial = nullptr,
+                       const T* dotv = nullptr) {
   if (g_wht_nt[sizeof(T) == 8 ? 1 : 0] == 1024)
-    launch_wht_nt<T, MODE, 1024>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial);
+    launch_wht_nt<T, MODE, 1024>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
   else
-    launch_wht_nt<T, MODE, 512>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial);
+    launch_wht_nt<T, MODE, 512>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
 }
 
 static MxArgs wht_args(const Grid& g, int64_t e_sx, int64_t e_sy, int* n_wg) {
@@ -580,6 +588,28 @@ int op0_wht_ydot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T
 }
 template int op0_wht_ydot<double>(const Grid&, int64_t, int64_t, const double*, const double*, const uint8_t*,
                                   const double*, double*, double*, int, hipStream_t);
+
+template <typename T>
+int op0_wht_jac_add_dot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E, const uint8_t* cons,
+                        const T* x, const T* b, const T* Dinv, const double* omega, double wscale, const T* xadd,
+                        T* y, const T* dotv, double* partial, int max_partial, hipStream_t s) {
+  ET_REQUIRE(wht_supported(g, sizeof(T)), "transformed-basis operator: 3-D grids with nelz <= 128 only");
+  int n_wg = 0;
+  const MxArgs A = wht_args(g, e_sx, e_sy, &n_wg);
+  ET_REQUIRE(partial != nullptr && n_wg <= max_partial, "transformed-basis operator: partial buffer too small");
+  ET_REQUIRE(dotv != nullptr && xadd != nullptr, "transformed-basis operator: null operand");
+  const size_t lds = wht_lds_values(g) * sizeof(T);
+  launch_wht<T, TM_JAC_ADD_DOT>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
+  ET_CHECK_LAUNCH();
+  return ET_OK;
+}
+template int op0_wht_jac_add_dot<float>(const Grid&, int64_t, int64_t, const float*, const float*, const uint8_t*,
+                                        const float*, const float*, const float*, const double*, double,
+                                        const float*, float*, const float*, double*, int, hipStream_t);
+template int op0_wht_jac_add_dot<double>(const Grid&, int64_t, int64_t, const double*, const double*,
+                                         const uint8_t*, const double*, const double*, const double*,
+                                         const double*, double, const double*, double*, const double*, double*,
+                                         int, hipStream_t);
 
 template <typename T>
 int op0_wht(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E,

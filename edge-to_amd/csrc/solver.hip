@@ -235,6 +235,9 @@ __global__ __launch_bounds__(256) void opS_row_kernel(const Grid g, const T* __r
 // coarse levels up to this size use opS_row_kernel (32-bit stencil offsets: also below 2^31 / 126 nodes)
 static int64_t g_row_stencil_max_nodes = 300000;
 void set_row_stencil_nodes(int64_t n) { g_row_stencil_max_nodes = n; }
+static int g_fused_rz = 1;
+void set_fused_rz(int on) { g_fused_rz = on ? 1 : 0; }
+int get_fused_rz() { return g_fused_rz; }
 int64_t get_row_stencil_nodes() { return g_row_stencil_max_nodes; }
 
 // y = omega*wscale * Dinv * b  (first Jacobi sweep from a zero guess)
@@ -275,32 +278,34 @@ __global__ void stencil_dinv_kernel(int64_t N, const double* __restrict__ S, T* 
   for (int r = 0; r < DIM; ++r) out[n * DIM + r] = (T)(1.0 / S[((int64_t)(oc * DIM + r) * DIM + r) * N + n]);
 }
 
-// PCG: x += alpha p; r -= alpha q (alpha = rho / pq); optionally a working-
-// precision copy of r for the V-cycle; per-block partial sums of r.r.  A no-op
-// once the loop has stopped (SC_STOP).
+// PCG: r -= alpha q (alpha = rho / pq); optionally a working-precision copy of
+// r for the V-cycle; per-block partial sums of r.r.  x += alpha p is deferred
+// to the p update, which reads p anyway: alpha goes to SC_ALPHA (zero when
+// this update did not run).  A no-op once the loop has stopped (SC_STOP).
 template <typename TB>
 __global__ __launch_bounds__(256) void pcg_update_xr_kernel(int64_t n, double* __restrict__ sc, int cur,
                                                             const double* __restrict__ pq_part, int n_pq,
-                                                            const double* __restrict__ p,
                                                             const double* __restrict__ q,
-                                                            double* __restrict__ x,
                                                             double* __restrict__ r,
                                                             TB* __restrict__ rb,
                                                             double* __restrict__ partial) {
   if (sc[SC_STOP] != 0.0) {
     if (threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[SC_ALPHA] = 0.0;
     return;
   }
   // p.q from its partials: every block sums them in the same fixed order
   double pq = 0.0;
   for (int t = threadIdx.x; t < n_pq; t += blockDim.x) pq += pq_part[t];
   pq = block_sum256_all(pq);
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc[SC_PQ] = pq;
   const double alpha = sc[SC_RHO + cur] / pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[SC_PQ] = pq;
+    sc[SC_ALPHA] = alpha;
+  }
   double s = 0.0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
-    x[t] += alpha * p[t];
     const double rv = r[t] - alpha * q[t];
     r[t] = rv;
     if (rb != nullptr) rb[t] = (TB)rv;
@@ -323,37 +328,50 @@ __global__ __launch_bounds__(256) void pcg_dot_kernel(int64_t n, const double* _
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// p = z + beta p, beta = rho_new / rho (Fletcher-Reeves); p = z when first.
-// rho_new = r.z is summed from its partials by every block in the same fixed
-// order and stored in slot SC_RHO + nxt; the previous rho is in the other slot.
+// x += alpha p (the deferred update of this iteration, alpha from SC_ALPHA),
+// then p = z + beta p, beta = rho_new / rho (Fletcher-Reeves); p = z and no x
+// update when first.  rho_new = r.z is summed from its partials by every block
+// in the same fixed order and stored in slot SC_RHO + nxt; the previous rho is
+// in the other slot.
 template <typename TZ>
 __global__ __launch_bounds__(256) void pcg_update_p_kernel(int64_t n, double* __restrict__ sc, int nxt,
                                                            const double* __restrict__ rz_part, int n_rz,
                                                            const TZ* __restrict__ z,
-                                                           double* __restrict__ p, int first) {
+                                                           double* __restrict__ p, double* __restrict__ x,
+                                                           int first) {
   double rz = 0.0;
   for (int t = threadIdx.x; t < n_rz; t += blockDim.x) rz += rz_part[t];
   rz = block_sum256_all(rz);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[SC_RHO + nxt] = rz;
   const double beta = first ? 0.0 : rz / sc[SC_RHO + (nxt ^ 1)];
-  // four consecutive values per step (z and p are 16-byte aligned allocations)
+  const double alpha = first ? 0.0 : sc[SC_ALPHA];
+  const bool upd_x = alpha != 0.0;  // zero: the x/r update of this iteration did not run
+  // four consecutive values per step (z, p and x are 16-byte aligned allocations)
   const int64_t n4 = n >> 2;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4;
        t += (int64_t)gridDim.x * blockDim.x) {
     TZ zv[4];
-    double pv[4];
+    double pv[4], xv[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) zv[u] = z[4 * t + u];
     if (!first) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) pv[u] = p[4 * t + u];
     }
+    if (upd_x) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xv[u] = x[4 * t + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) x[4 * t + u] = xv[u] + alpha * pv[u];
+    }
 #pragma unroll
     for (int u = 0; u < 4; ++u) p[4 * t + u] = first ? (double)zv[u] : (double)zv[u] + beta * pv[u];
   }
   for (int64_t t = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * blockDim.x)
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (upd_x) x[t] += alpha * p[t];
     p[t] = first ? (double)z[t] : (double)z[t] + beta * p[t];
+  }
 }
 
 // Convergence / breakdown test of one iteration, evaluated on the device so the
@@ -527,6 +545,16 @@ int MGHierarchy<T>::vcycle(int l, MGShared& sh, hipStream_t s) {
       if (it + 1 < n_smooth) {
         ET_TRY(op_launch<T>(L, sh, OP_JAC, L.e, L.r, om, ws, nullptr, L.t, s));
         std::swap(L.e, L.t);
+      } else if (l == 0 && pass + 1 == passes && sh.want_dot) {
+        // the preconditioned residual z is final here: r.z partials in the same pass
+        sh.prof->begin(ET_PROF_L0_JACADD, s);
+        const int st = op0_wht_jac_add_dot<T>(L.g, (int64_t)L.g.ey * L.g.ez, L.g.ez, sh.K0<T>(), sh.E<T>(),
+                                              sh.cons, L.e, L.r, L.Dinv, om, ws, L.x, L.t, L.b,
+                                              sh.dot_part, kRedBlocks, s);
+        sh.prof->end(s);
+        ET_TRY(st);
+        sh.dot_done = true;
+        std::swap(L.x, L.t);
       } else {
         ET_TRY(op_launch<T>(L, sh, OP_JAC_ADD, L.e, L.r, om, ws, L.x, L.t, s));
         std::swap(L.x, L.t);
@@ -896,13 +924,18 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
   // partial: first-stage sums of p.q and r.z; partial2: of r.r
   double* partial2 = partial + kRedBlocks;
   pcg_dot_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, r, z, partial);
-  pcg_update_p_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, 0, partial, kRedBlocks, z, p_, 1);
+  pcg_update_p_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, 0, partial, kRedBlocks, z, p_, x, 1);
   prof.end(s);
   ET_CHECK_LAUNCH();
   // the fused q = A p, p.q path exists for the transformed-basis operator
   const bool fused_pq = sh.tiled && g0.dim == 3 && get_wht() && wht_supported(g0, sizeof(double)) &&
                         wht_ydot_partials(g0) <= kRedBlocks;
   const int n_pq_fused = fused_pq ? wht_ydot_partials(g0) : 0;
+  // FP32 V-cycles: the last fine-grid sweep (same operator) yields r.z, with r
+  // as the V-cycle's FP32 copy (the FP64 sweep has no registers to spare)
+  const bool fused_rz = fused_pq && sizeof(T) == sizeof(float) && get_fused_rz() &&
+                        wht_supported(g0, sizeof(T)) && !sh.mfma && h.n_smooth > 0;
+  const int n_rz_fused = fused_rz ? wht_ydot_partials(g0) : kRedBlocks;
   int last = 0;
   auto read_ring = [&](int k, double* R, int* st) -> bool {
     ET_CHECK_HIP(hipEventSynchronize(ring_ev[k & 1]));
@@ -931,7 +964,7 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
       prof.begin(ET_PROF_PCG_VEC, s);
       ET_TRY(dot_partials(n_dof, p_, q, partial, s));
     }
-    pcg_update_xr_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, cur, partial, n_pq, p_, q, x, r, rb,
+    pcg_update_xr_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, cur, partial, n_pq, q, r, rb,
                                                        partial2);
     ET_CHECK_LAUNCH();
     pcg_check_kernel<<<1, 256, 0, s>>>(sh.scal, partial2, kRedBlocks, tol, min_omega, d_omega,
@@ -945,10 +978,20 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
       stopped = true;
       break;
     }
-    ET_TRY(precondition(h, true, &z));
+    // r.z partials from the V-cycle's last fine sweep where it can produce them
+    sh.want_dot = fused_rz;
+    sh.dot_part = partial;
+    sh.dot_done = false;
+    const int pst = precondition(h, true, &z);
+    sh.want_dot = false;
+    ET_TRY(pst);
     prof.begin(ET_PROF_PCG_VEC, s);
-    pcg_dot_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, r, z, partial);
-    pcg_update_p_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, it & 1, partial, kRedBlocks, z, p_, 0);
+    int n_rz = n_rz_fused;
+    if (!sh.dot_done) {
+      pcg_dot_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, r, z, partial);
+      n_rz = kRedBlocks;
+    }
+    pcg_update_p_kernel<T><<<kRedBlocks, 256, 0, s>>>(n_dof, sh.scal, it & 1, partial, n_rz, z, p_, x, 0);
     prof.end(s);
     ET_CHECK_LAUNCH();
   }

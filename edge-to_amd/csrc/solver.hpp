@@ -10,7 +10,7 @@ namespace et {
 
 // device scalar slots of the PCG loop
 // SC_RHO, SC_RHO + 1: r.z of consecutive PCG iterations (slots alternate)
-enum { SC_BB = 0, SC_RHO, SC_RHO_ALT, SC_PQ, SC_RR, SC_OMEGA, SC_R, SC_STATUS, SC_STOP, SC_COUNT };
+enum { SC_BB = 0, SC_RHO, SC_RHO_ALT, SC_PQ, SC_RR, SC_OMEGA, SC_R, SC_STATUS, SC_STOP, SC_ALPHA, SC_COUNT };
 
 // Event-based per-kernel-class timing (enabled on demand).
 struct Prof {
@@ -70,6 +70,12 @@ struct MGShared {
   const uint8_t* cons = nullptr;
   double* scal = nullptr;      // device scalars (SC_*)
   const Level1MF* l1 = nullptr;  // matrix-free first coarse level, when enabled
+  // When set, the last fine-grid sweep of the V-cycle also writes the
+  // per-workgroup partial sums of b.z (b the fine-grid right-hand side in the
+  // V-cycle's precision, z its output) to dot_part and sets dot_done.
+  bool want_dot = false;
+  double* dot_part = nullptr;
+  bool dot_done = false;
   template <typename T>
   const T* K0() const;
   template <typename T>
@@ -153,6 +159,8 @@ struct MGSolver {
 };
 
 void set_row_stencil_nodes(int64_t n);
+void set_fused_rz(int on);
+int get_fused_rz();
 int64_t get_row_stencil_nodes();
 
 }  // namespace et

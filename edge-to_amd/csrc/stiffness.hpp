@@ -43,6 +43,12 @@ template <typename T>
 int op0_wht_ydot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E, const uint8_t* cons,
                  const T* x, T* y, double* partial, int max_partial, hipStream_t s);
 int wht_ydot_partials(const Grid& g);
+// Final post-smoothing sweep y = xadd + Jacobi(x; b) with the per-workgroup
+// partial sums of dotv.y (same partial count as op0_wht_ydot).
+template <typename T>
+int op0_wht_jac_add_dot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E, const uint8_t* cons,
+                        const T* x, const T* b, const T* Dinv, const double* omega, double wscale, const T* xadd,
+                        T* y, const T* dotv, double* partial, int max_partial, hipStream_t s);
 void set_wht(int on);
 int get_wht();
 void set_wht_wgs(int n);

@@ -69,7 +69,11 @@ const char* et_version(void);
  *   stencil with one thread per (node, component) instead of one per node.
  * ET_TUNE_LEVEL1_MF: 1 (default) = the first coarse level of 3-D hierarchies with even
  *   element counts is applied matrix-free from the element densities; 0 = stored stencil.
- *   Read by et_mg_create. */
+ *   Read by et_mg_create.
+ * ET_TUNE_FUSED_RZ: 1 (default) = in PCG with FP32 V-cycles on the transformed-basis
+ *   operator, the last fine-grid sweep of each V-cycle also sums r.z per workgroup,
+ *   r taken as the V-cycle's FP32 copy (no separate pass over r and z); 0 = separate
+ *   reduction with r in FP64.  Read by each solve. */
 enum {
   ET_TUNE_TILED_JC = 0,
   ET_TUNE_MFMA_JC = 1,
@@ -84,7 +88,8 @@ enum {
   ET_TUNE_WHT_THREADS_F32 = 10,
   ET_TUNE_WHT_THREADS_F64 = 11,
   ET_TUNE_ROW_STENCIL_NODES = 12,
-  ET_TUNE_LEVEL1_MF = 13
+  ET_TUNE_LEVEL1_MF = 13,
+  ET_TUNE_FUSED_RZ = 14
 };
 int et_tune_set(int param, int value);
 int et_tune_get(int param);

@@ -205,3 +205,55 @@ def test_coarse_solve_matches_dense(coarse_lib, precision, nel):
     L.call("et_mg_coarse_solve", solver._handle, L.ptr(bt), L.ptr(xt))
     err = np.abs(xt.cpu().numpy() - xd).max() / np.abs(xd).max()
     assert err < (1e-5 if precision == "mixed" else 1e-9), err
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("maxiter", [1000, 7])
+@pytest.mark.parametrize("fused_rz", [1, 0])
+def test_pcg_vector_updates(precision, maxiter, fused_rz):
+    """PCG with r.z summed inside the last fine-grid sweep (ET_TUNE_FUSED_RZ = 1, FP32 V-cycles)
+    or by its own pass (0), and with x += alpha p deferred to the p update: the returned x must satisfy
+    |f - A x| / |f| = the reported residual, whether the loop ends on the tolerance or on
+    maxiter, and both settings give the same iterates."""
+    ET = _edgeto()
+    from edgeto_amd import _lib as L
+    nel = (24, 12, 12)
+    old = L.lib().et_tune_get(14)
+    L.call("et_tune_set", 14, fused_rz)
+    try:
+        mesh, kernel, filt, clamp, load = _cantilever_3d(ET, nel)
+        rng = np.random.default_rng(11)
+        E = rng.uniform(0.05, 1.0, mesh.n_el)
+        kernel.set_rho(torch.tensor(E, device="cuda"))
+        tp = ET.CuTopOpt(mesh, ET.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=0.3), kernel,
+                         None, filt, verbose=False)
+        tp.add_BC_nodal(clamp, np.array([[1, 1, 1]]))
+        tp.add_F_nodal(load, np.array([[0.0, -1.0 / len(load), 0.0]]))
+        solver = ET.CuMultiGrid(mesh, kernel, tol=1e-10, maxiter=maxiter, n_smooth=2, n_level=2,
+                                precision=precision)
+        x, res = solver.solve(tp.f, use_last=False)
+        iters = solver.last_iterations
+    finally:
+        L.call("et_tune_set", 14, old)
+    prob = _oracle_problem(nel, 0.3, 2.0, clamp, load, -1.0 / len(load))
+    A = oops.assemble(prob.K0, prob.elements, E, prob.cons, prob.n_dof)
+    xg = x.cpu().numpy()
+    true_res = np.linalg.norm(prob.f - A @ xg) / np.linalg.norm(prob.f)
+    if maxiter == 7:
+        assert iters == 7
+        assert res > 1e-10
+    else:
+        assert iters < maxiter and res < 1e-10
+    # the recursively updated residual tracks the true one
+    assert abs(true_res - res) <= 1e-6 * res + 2e-11, (true_res, res)
+    key = (precision, maxiter)
+    _PCG_RUNS.setdefault(key, []).append((iters, xg))
+    if len(_PCG_RUNS[key]) == 2:
+        (i0, x0), (i1, x1) = _PCG_RUNS[key]
+        assert abs(i0 - i1) <= 1
+        if i0 == i1:
+            tol = 1e-6 if precision == "mixed" else 1e-9
+            assert np.linalg.norm(x0 - x1) / np.linalg.norm(x0) < tol
+
+
+_PCG_RUNS = {}
