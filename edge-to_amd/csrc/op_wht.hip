@@ -103,6 +103,22 @@ __device__ inline double wht_uniform(double v) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// Global accesses at a 32-bit element index from a kernel-argument base: the
+// byte offset is formed in 32 bits and zero-extended, so the access takes the
+// scalar-base + vector-offset form with no 64-bit address arithmetic per access
+// (the host checks that every byte offset fits in 32 bits).
+// Element i + k: the constant k goes into the instruction's offset field.
+template <typename T>
+__device__ inline T ld32(const T* __restrict__ p, int i, int k = 0) {
+  const char* a = reinterpret_cast<const char*>(p) + (size_t)((unsigned)i * (unsigned)sizeof(T));
+  return *reinterpret_cast<const T*>(a + (size_t)k * sizeof(T));
+}
+template <typename T>
+__device__ inline void st32(T* __restrict__ p, int i, int k, T v) {
+  char* a = reinterpret_cast<char*>(p) + (size_t)((unsigned)i * (unsigned)sizeof(T));
+  *reinterpret_cast<T*>(a + (size_t)k * sizeof(T)) = v;
+}
+
 // 4-point transform over (hx, hz): F[px + 2 pz] = sum (-1)^(px hx + pz hz) u[hx][hz]
 template <typename T>
 __device__ inline void wht_face(const T u00, const T u10, const T u01, const T u11, T* F) {
@@ -143,7 +159,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
   T* sg = se + tc * nelz;                 // G, 63 entries
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wave_u = __builtin_amdgcn_readfirstlane(wave);
-  // 32-bit DOF offsets: the host checks 3 * n_nodes < 2^31
+  // 32-bit DOF indices and byte offsets: the host checks 3 * n_nodes < 2^31 and 3 * n_nodes * sizeof(T) < 2^32
   const int plane3 = nx * nz * 3;
   // Without constraints every constraint-byte read goes to byte 0 of E and is
   // masked to zero.  Both candidates are kernel arguments, so the loads stay
@@ -255,12 +271,12 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
             if (kZero) {
-              pu[m][r] = b[d + r];
-              pdv[kZero ? m : 0][r] = Dinv[d + r];
+              pu[m][r] = ld32(b, d, r);
+              pdv[kZero ? m : 0][r] = ld32(Dinv, d, r);
             } else {
-              pu[m][r] = x[d + r];
+              pu[m][r] = ld32(x, d, r);
             }
-            pk[m][r] = cp[(d + r) & cmask] & cbyte;
+            pk[m][r] = ld32(cp, d & cmask, r) & cbyte;
           }
         }
       }
@@ -285,7 +301,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
       for (int m = 0; m < kWEStage; ++m) {
         const int c = c0 + ecl[m];
         pev[m] = lin && ecl[m] < tc && c >= 0 && c < nelx;
-        pe[m] = E[pev[m] ? c * (int)A.e_sx + ey * (int)A.e_sy + eez[m] : 0];
+        pe[m] = ld32(E, pev[m] ? c * (int)A.e_sx + ey * (int)A.e_sy + eez[m] : 0);
       }
     };
     auto store_e = [&]() {
@@ -349,11 +365,11 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
         dd[q] = act[q] ? jb3 + osl[q] * 3 : 0;
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-          cq[q][r] = cp[(dd[q] + r) & cmask] & cbyte;
-          bq[q][r] = (!kProd && !kZero) ? b[dd[q] + r] : T(0);
-          dq[q][r] = kJac ? Dinv[dd[q] + r] : T(0);
-          aq[q][r] = kAdd ? xadd[dd[q] + r] : T(0);
-          if (MODE == TM_JAC_ADD_DOT) vq[MODE == TM_JAC_ADD_DOT ? q : 0][r] = dotv[dd[q] + r];
+          cq[q][r] = ld32(cp, dd[q] & cmask, r) & cbyte;
+          bq[q][r] = (!kProd && !kZero) ? ld32(b, dd[q], r) : T(0);
+          dq[q][r] = kJac ? ld32(Dinv, dd[q], r) : T(0);
+          aq[q][r] = kAdd ? ld32(xadd, dd[q], r) : T(0);
+          if (MODE == TM_JAC_ADD_DOT) vq[MODE == TM_JAC_ADD_DOT ? q : 0][r] = ld32(dotv, dd[q], r);
         }
       }
 
@@ -434,40 +450,50 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
             const int nl = onl[q], k = ok_[q];
             const bool wv = k < 64 * spc;  // contribution slot written by a lane
             const bool seam = k > 0 && (k & 63) == 0;
+            // both slot kinds are read at clamped (always written) indices and
+            // selected afterwards, so the reads do not wait one by one
+            const int kw = min(k, 64 * spc - 1);
+            const int ks = max((k >> 6) - 1, 0);
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-              const int d = dd[q] + r;
-              T ax0 = T(0);
-              if (wv) ax0 = sw[((r * 2) * tc + nl) * nz + k] + sw[((r * 2 + 1) * tc + nl - 1) * nz + k];
-              if (seam)
-                ax0 += ss[((r * 2) * tc + nl) * 2 + (k >> 6) - 1] + ss[((r * 2 + 1) * tc + nl - 1) * 2 + (k >> 6) - 1];
+              const int d = dd[q];
+              const T s0 = sw[((r * 2) * tc + nl) * nz + kw] + sw[((r * 2 + 1) * tc + nl - 1) * nz + kw];
+              const T s1 = ss[((r * 2) * tc + nl) * 2 + ks] + ss[((r * 2 + 1) * tc + nl - 1) * 2 + ks];
+              const T ax0 = (wv ? s0 : T(0)) + (seam ? s1 : T(0));
               const bool cr = cq[q][r] != 0;
               T xv = xq[q][r];  // staged value: zero on constrained DOFs
               T vv;
-              if (kZero) {
+              // unconstrained row
+              if (kZero)
                 // x0 = w D^-1 b is the staged value, so x0 + w D^-1 (b - A x0) = 2 x0 - w D^-1 A x0
-                // (b itself is read only on constrained DOFs, where A x0 = x0)
-                if (cr) {
-                  const T bv = b[d];
-                  xv = w * dq[q][r] * bv;
-                  vv = xv + w * dq[q][r] * (bv - xv);
+                vv = xv + xv - w * dq[q][r] * ax0;
+              else if (kProd)
+                vv = ax0;
+              else if (MODE == TM_RES)
+                vv = bq[q][r] - ax0;
+              else
+                vv = xv + w * dq[q][r] * (bq[q][r] - ax0);
+              if (cr) {
+                // identity row (A x = x): the value is read here, where it is used, so
+                // only waves holding a constrained DOF wait for it
+                if (kZero) {
+                  const T bv = ld32(b, d, r);
+                  const T x0 = w * dq[q][r] * bv;
+                  vv = x0 + w * dq[q][r] * (bv - x0);
                 } else {
-                  vv = xv + xv - w * dq[q][r] * ax0;
+                  xv = ld32(x, d, r);
+                  if (kProd)
+                    vv = xv;
+                  else if (MODE == TM_RES)
+                    vv = bq[q][r] - xv;
+                  else
+                    vv = xv + w * dq[q][r] * (bq[q][r] - xv);
                 }
-              } else {
-                if (cr) xv = x[d];
-                const T ax = cr ? xv : ax0;
-                if (kProd)
-                  vv = ax;
-                else if (MODE == TM_RES)
-                  vv = bq[q][r] - ax;
-                else
-                  vv = xv + w * dq[q][r] * (bq[q][r] - ax);
               }
               if (kAdd) vv += aq[q][r];
               if (MODE == TM_YDOT) dot += (double)xv * (double)vv;
               if (MODE == TM_JAC_ADD_DOT) dot += (double)vq[MODE == TM_JAC_ADD_DOT ? q : 0][r] * (double)vv;
-              y[d] = vv;
+              st32(y, d, r, vv);
             }
           }
         }
@@ -512,7 +538,10 @@ int get_wht_wgs() { return g_wht_wgs; }
 constexpr size_t kWMaxLds = 160 * 1024;
 
 bool wht_supported(const Grid& g, size_t elem_bytes) {
-  if (g.dim != 3 || g.ez > 128 || 3 * g.n_nodes() >= ((int64_t)1 << 31)) return false;
+  // 32-bit element indices and 32-bit byte offsets
+  if (g.dim != 3 || g.ez > 128 || 3 * g.n_nodes() >= ((int64_t)1 << 31) ||
+      3 * g.n_nodes() * (int64_t)elem_bytes >= ((int64_t)1 << 32) || g.n_elem() * (int64_t)elem_bytes >= ((int64_t)1 << 32))
+    return false;
   const int tc = wht_tc(g);
   return (tc + 1) * g.nz() <= kWStageCap && tc * g.ez <= kWECap && (tc - 1) * g.nz() <= kWNodeCap &&
          wht_lds_values(g) * elem_bytes <= kWMaxLds;
