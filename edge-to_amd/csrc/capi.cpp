@@ -85,8 +85,12 @@ int et_tune_set(int param, int value) {
       ET_REQUIRE(value >= 1 && value <= 8, "et_tune_set: workgroups per CU must be in [1, 8]");
       et::set_mx_wgs(param == ET_TUNE_MX_WGS_F64, value);
     } else if (param == ET_TUNE_WHT_THREADS_F32 || param == ET_TUNE_WHT_THREADS_F64) {
-      ET_REQUIRE(value == 512 || value == 1024, "et_tune_set: workgroup size must be 512 or 1024");
+      ET_REQUIRE(value == 256 || value == 512 || value == 1024,
+                 "et_tune_set: workgroup size must be 256, 512 or 1024");
       et::set_wht_threads(param == ET_TUNE_WHT_THREADS_F64, value);
+    } else if (param == ET_TUNE_WHT_SLICES_F32 || param == ET_TUNE_WHT_SLICES_F64) {
+      ET_REQUIRE(value == 8 || value == 16, "et_tune_set: slices per layer must be 8 or 16");
+      et::set_wht_slices(param == ET_TUNE_WHT_SLICES_F64, value);
     } else if (param == ET_TUNE_LEVEL1_MF) {
       et::set_level1_mf(value);
     } else if (param == ET_TUNE_FUSED_RZ) {
@@ -117,6 +121,8 @@ int et_tune_get(int param) {
   if (param == ET_TUNE_COARSE_LIB) return et::get_coarse_lib();
   if (param == ET_TUNE_LEVEL1_MF) return et::get_level1_mf();
   if (param == ET_TUNE_FUSED_RZ) return et::get_fused_rz();
+  if (param == ET_TUNE_WHT_SLICES_F32 || param == ET_TUNE_WHT_SLICES_F64)
+    return et::get_wht_slices(param == ET_TUNE_WHT_SLICES_F64);
   if (param == ET_TUNE_ROW_STENCIL_NODES) return (int)et::get_row_stencil_nodes();
   if (param == ET_TUNE_WHT_THREADS_F32 || param == ET_TUNE_WHT_THREADS_F64)
     return et::get_wht_threads(param == ET_TUNE_WHT_THREADS_F64);

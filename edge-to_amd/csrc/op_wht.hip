@@ -43,20 +43,21 @@
 
 namespace et {
 
-// Per-workgroup capacities (independent of the workgroup size): 16 slices of
-// 64 elements per layer, (TC+1)*nz <= 1536 staged nodes, TC*nelz <= 1024 E
-// values, (TC-1)*nz <= 1024 output nodes.
-constexpr int kWStageCap = 1536, kWECap = 1024, kWNodeCap = 1024;
 constexpr int kWG = 63;  // transformed blocks M = 1..7, 3 x 3 each
 
-// workgroup of NT threads (512 or 1024)
-template <int NT>
+// Workgroup of NT threads on a tile of NS slices of 64 elements per layer
+// (NS = 16: TC = 16 element columns for nelz <= 64, 8 for nelz <= 128; NS = 8:
+// half of that).  Per-workgroup capacities: (TC+1)*nz <= 96 NS staged nodes,
+// TC*nelz <= 64 NS E values, (TC-1)*nz <= 64 NS output nodes.
+template <int NT, int NS>
 struct WCfg {
   static constexpr int waves = NT / 64;
-  static constexpr int slots = 16 / waves;                     // elements per thread per layer
-  static constexpr int stage = (kWStageCap + NT - 1) / NT;     // staged nodes per thread
-  static constexpr int estage = kWECap / NT;                   // staged E per thread
-  static constexpr int nodes = kWNodeCap / NT;                 // output nodes per thread
+  static constexpr int stage_cap = 96 * NS, e_cap = 64 * NS, node_cap = 64 * NS;
+  static constexpr int slots = NS / waves;                   // elements per thread per layer
+  static constexpr int stage = (stage_cap + NT - 1) / NT;    // staged nodes per thread
+  static constexpr int estage = e_cap / NT;                  // staged E per thread
+  static constexpr int nodes = node_cap / NT;                // output nodes per thread
+  static_assert(slots >= 1 && estage >= 1 && nodes >= 1, "tile smaller than the workgroup");
 };
 
 // structural non-zeros of G[M][c][d] (M = 1..7)
@@ -70,10 +71,10 @@ __host__ __device__ constexpr bool wht_nz(int M, int c, int d) {
   return bits == 1 ? (pair & M) != 0 : pair != M;
 }
 
-__host__ inline int wht_tc(const Grid& g) { return g.ez <= 64 ? 16 : 8; }
+__host__ inline int wht_tc(const Grid& g, int ns) { return g.ez <= 64 ? ns : ns / 2; }
 
-__host__ inline size_t wht_lds_values(const Grid& g) {
-  const int tc = wht_tc(g);
+__host__ inline size_t wht_lds_values(const Grid& g, int ns) {
+  const int tc = wht_tc(g, ns);
   const size_t nz = g.nz(), span = (size_t)(tc + 1) * nz;
   return 6 * span                 // two staged node planes
          + 6 * (size_t)tc * nz    // node contributions [3][2][tc][nz]
@@ -129,8 +130,16 @@ __device__ inline void wht_face(const T u00, const T u10, const T u01, const T u
   F[3] = a1 - b1;
 }
 
-template <typename T, int MODE, int NT>
-__global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __restrict__ K0,
+// FP32 on 8-slice tiles of 512 threads runs two workgroups per compute unit:
+// at most 128 VGPRs (4 waves per SIMD)
+template <typename T, int NT, int NS>
+constexpr int wht_min_waves() {
+  return (sizeof(T) == 4 && NT == 512 && NS == 8) ? 4 : 1;
+}
+
+template <typename T, int MODE, int NT, int NS>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(wht_min_waves<T, NT, NS>())))
+void op0_wht_kernel(const MxArgs A, const T* __restrict__ K0,
                                                             const T* __restrict__ E,
                                                             const uint8_t* __restrict__ cons,
                                                             const T* __restrict__ x,
@@ -150,7 +159,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
   const int nx = g.nx(), ny = g.ny(), nz = g.nz();
   const int nelx = g.ex, nely = g.ey, nelz = g.ez;
   const int spc = nelz <= 64 ? 1 : 2;  // 64-element slices per element column
-  const int tc = 16 / spc;             // element columns per x tile
+  const int tc = NS / spc;             // element columns per x tile
   const int span = (tc + 1) * nz;
   T* su = reinterpret_cast<T*>(lds_raw);  // [2 slots][3 comps][span]
   T* sw = su + 6 * span;                  // [3 comps][2 hx][tc][nz]
@@ -169,10 +178,11 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
   const int cmask = cons != nullptr ? ~0 : 0;
   const uint8_t cbyte = cons != nullptr ? (uint8_t)0xff : (uint8_t)0;
   constexpr int kWThreads = NT;
-  constexpr int kWSlots = WCfg<NT>::slots;
-  constexpr int kWStage = WCfg<NT>::stage;
-  constexpr int kWEStage = WCfg<NT>::estage;
-  constexpr int kWNodes = WCfg<NT>::nodes;
+  using Cfg = WCfg<NT, NS>;
+  constexpr int kWSlots = Cfg::slots;
+  constexpr int kWStage = Cfg::stage;
+  constexpr int kWEStage = Cfg::estage;
+  constexpr int kWNodes = Cfg::nodes;
 
   // G[M][c][d] from K0 (corner order of K0 is the H8 local order)
   if (tid < kWG) {
@@ -211,7 +221,7 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
   int scl[kWSlots], sez[kWSlots];
 #pragma unroll
   for (int m = 0; m < kWSlots; ++m) {
-    const int s = wave + WCfg<NT>::waves * m;
+    const int s = wave + Cfg::waves * m;
     scl[m] = s / spc;
     sez[m] = 64 * (s % spc) + lane;
   }
@@ -525,10 +535,22 @@ __global__ __launch_bounds__(NT) void op0_wht_kernel(const MxArgs A, const T* __
 
 static int g_wht_enable = 1;
 static int g_wht_wgs = 1;
-static int g_wht_nt[2] = {1024, 512};  // workgroup size [float, double]
+// tile per precision [float, double]: workgroup size and slices per layer of
+// the sweeps (residual and Jacobi modes); y = A x products always take
+// 16-slice tiles (fastest in both precisions)
+static int g_wht_nt[2] = {1024, 512};
+static int g_wht_ns[2] = {16, 8};
 
-void set_wht_threads(int is_double, int nt) { g_wht_nt[is_double ? 1 : 0] = nt == 1024 ? 1024 : 512; }
+static int wht_ns(size_t elem_bytes, int mode) {
+  return (mode == TM_Y || mode == TM_YDOT) ? 16 : g_wht_ns[elem_bytes == 8 ? 1 : 0];
+}
+
+void set_wht_threads(int is_double, int nt) {
+  g_wht_nt[is_double ? 1 : 0] = (nt == 1024 || nt == 256) ? nt : 512;
+}
 int get_wht_threads(int is_double) { return g_wht_nt[is_double ? 1 : 0]; }
+void set_wht_slices(int is_double, int ns) { g_wht_ns[is_double ? 1 : 0] = ns == 8 ? 8 : 16; }
+int get_wht_slices(int is_double) { return g_wht_ns[is_double ? 1 : 0]; }
 
 void set_wht(int on) { g_wht_enable = on ? 1 : 0; }
 int get_wht() { return g_wht_enable; }
@@ -542,9 +564,13 @@ bool wht_supported(const Grid& g, size_t elem_bytes) {
   if (g.dim != 3 || g.ez > 128 || 3 * g.n_nodes() >= ((int64_t)1 << 31) ||
       3 * g.n_nodes() * (int64_t)elem_bytes >= ((int64_t)1 << 32) || g.n_elem() * (int64_t)elem_bytes >= ((int64_t)1 << 32))
     return false;
-  const int tc = wht_tc(g);
-  return (tc + 1) * g.nz() <= kWStageCap && tc * g.ez <= kWECap && (tc - 1) * g.nz() <= kWNodeCap &&
-         wht_lds_values(g) * elem_bytes <= kWMaxLds;
+  for (const int ns : {16, g_wht_ns[elem_bytes == 8 ? 1 : 0]}) {
+    const int tc = wht_tc(g, ns);
+    if (!((tc + 1) * g.nz() <= 96 * ns && tc * g.ez <= 64 * ns && (tc - 1) * g.nz() <= 64 * ns &&
+          wht_lds_values(g, ns) * elem_bytes <= kWMaxLds))
+      return false;
+  }
+  return true;
 }
 
 static int wht_device_cus() {
@@ -558,48 +584,63 @@ static int wht_device_cus() {
   return cus;
 }
 
-template <typename T, int MODE, int NT>
-static void launch_wht_nt(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const T* K0, const T* E,
-                          const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
-                          double wscale, const T* xadd, T* y, double* partial, const T* dotv) {
+template <typename T, int MODE, int NT, int NS>
+static void launch_wht_cfg(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const T* K0, const T* E,
+                           const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
+                           double wscale, const T* xadd, T* y, double* partial, const T* dotv) {
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&op0_wht_kernel<T, MODE, NT>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&op0_wht_kernel<T, MODE, NT, NS>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWMaxLds);
     attr_set = true;
   }
-  op0_wht_kernel<T, MODE, NT><<<n_wg, NT, lds, s>>>(A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial,
-                                                    dotv);
+  op0_wht_kernel<T, MODE, NT, NS><<<n_wg, NT, lds, s>>>(A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y,
+                                                        partial, dotv);
 }
 
+// 16 slices: 1024 or 512 threads; 8 slices: 512 or 256 threads
 template <typename T, int MODE>
 static void launch_wht(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const T* K0, const T* E,
                        const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
                        double wscale, const T* xadd, T* y, double* partial = nullptr,
                        const T* dotv = nullptr) {
-  if (g_wht_nt[sizeof(T) == 8 ? 1 : 0] == 1024)
-    launch_wht_nt<T, MODE, 1024>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
-  else
-    launch_wht_nt<T, MODE, 512>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
+  const int p = sizeof(T) == 8 ? 1 : 0;
+  if (wht_ns(sizeof(T), MODE) == 16) {
+    if (g_wht_nt[p] == 1024)
+      launch_wht_cfg<T, MODE, 1024, 16>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
+    else
+      launch_wht_cfg<T, MODE, 512, 16>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
+  } else {
+    if (g_wht_nt[p] == 256)
+      launch_wht_cfg<T, MODE, 256, 8>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
+    else
+      launch_wht_cfg<T, MODE, 512, 8>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
+  }
 }
 
-static MxArgs wht_args(const Grid& g, int64_t e_sx, int64_t e_sy, int* n_wg) {
+// x tiles, persistent grid (ET_TUNE_WHT_WGS workgroups per compute unit for
+// 16-slice tiles, twice as many for 8-slice tiles) and LDS bytes
+static MxArgs wht_args(const Grid& g, int64_t e_sx, int64_t e_sy, size_t elem_bytes, int mode, int* n_wg,
+                       size_t* lds) {
   MxArgs A;
   A.g = g;
   A.e_sx = e_sx;
   A.e_sy = e_sy;
-  const int tc = wht_tc(g);
+  const int ns = wht_ns(elem_bytes, mode);
+  const int tc = wht_tc(g, ns);
   A.n_xt = (g.nx() + tc - 2) / (tc - 1);
   A.nzg = 0;
   A.units = (int64_t)A.n_xt * g.ny();
-  const int64_t slots = (int64_t)wht_device_cus() * g_wht_wgs;
+  const int64_t slots = (int64_t)wht_device_cus() * g_wht_wgs * (16 / ns);
   *n_wg = (int)(A.units < slots ? A.units : slots);
+  *lds = wht_lds_values(g, ns) * elem_bytes;
   return A;
 }
 
-int wht_ydot_partials(const Grid& g) {
+int wht_partials(const Grid& g, size_t elem_bytes, int mode) {
   int n_wg = 0;
-  (void)wht_args(g, 0, 0, &n_wg);
+  size_t lds = 0;
+  (void)wht_args(g, 0, 0, elem_bytes, mode, &n_wg, &lds);
   return n_wg;
 }
 
@@ -608,9 +649,9 @@ int op0_wht_ydot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T
                  const T* x, T* y, double* partial, int max_partial, hipStream_t s) {
   ET_REQUIRE(wht_supported(g, sizeof(T)), "transformed-basis operator: 3-D grids with nelz <= 128 only");
   int n_wg = 0;
-  const MxArgs A = wht_args(g, e_sx, e_sy, &n_wg);
+  size_t lds = 0;
+  const MxArgs A = wht_args(g, e_sx, e_sy, sizeof(T), TM_YDOT, &n_wg, &lds);
   ET_REQUIRE(partial != nullptr && n_wg <= max_partial, "transformed-basis operator: partial buffer too small");
-  const size_t lds = wht_lds_values(g) * sizeof(T);
   launch_wht<T, TM_YDOT>(n_wg, lds, s, A, K0, E, cons, x, nullptr, nullptr, nullptr, 1.0, nullptr, y, partial);
   ET_CHECK_LAUNCH();
   return ET_OK;
@@ -624,10 +665,10 @@ int op0_wht_jac_add_dot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, 
                         T* y, const T* dotv, double* partial, int max_partial, hipStream_t s) {
   ET_REQUIRE(wht_supported(g, sizeof(T)), "transformed-basis operator: 3-D grids with nelz <= 128 only");
   int n_wg = 0;
-  const MxArgs A = wht_args(g, e_sx, e_sy, &n_wg);
+  size_t lds = 0;
+  const MxArgs A = wht_args(g, e_sx, e_sy, sizeof(T), TM_JAC_ADD_DOT, &n_wg, &lds);
   ET_REQUIRE(partial != nullptr && n_wg <= max_partial, "transformed-basis operator: partial buffer too small");
   ET_REQUIRE(dotv != nullptr && xadd != nullptr, "transformed-basis operator: null operand");
-  const size_t lds = wht_lds_values(g) * sizeof(T);
   launch_wht<T, TM_JAC_ADD_DOT>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
   ET_CHECK_LAUNCH();
   return ET_OK;
@@ -645,17 +686,9 @@ int op0_wht(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, co
             const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
             double wscale, const T* xadd, T* y, hipStream_t s) {
   ET_REQUIRE(wht_supported(g, sizeof(T)), "transformed-basis operator: 3-D grids with nelz <= 128 only");
-  MxArgs A;
-  A.g = g;
-  A.e_sx = e_sx;
-  A.e_sy = e_sy;
-  const int tc = wht_tc(g);
-  A.n_xt = (g.nx() + tc - 2) / (tc - 1);
-  A.nzg = 0;
-  A.units = (int64_t)A.n_xt * g.ny();
-  const size_t lds = wht_lds_values(g) * sizeof(T);
-  const int64_t slots = (int64_t)wht_device_cus() * g_wht_wgs;
-  const int n_wg = (int)(A.units < slots ? A.units : slots);
+  int n_wg = 0;
+  size_t lds = 0;
+  const MxArgs A = wht_args(g, e_sx, e_sy, sizeof(T), mode, &n_wg, &lds);
   switch (mode) {
     case TM_Y: launch_wht<T, TM_Y>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
     case TM_RES: launch_wht<T, TM_RES>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;

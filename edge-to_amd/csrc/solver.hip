@@ -929,13 +929,14 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
   ET_CHECK_LAUNCH();
   // the fused q = A p, p.q path exists for the transformed-basis operator
   const bool fused_pq = sh.tiled && g0.dim == 3 && get_wht() && wht_supported(g0, sizeof(double)) &&
-                        wht_ydot_partials(g0) <= kRedBlocks;
-  const int n_pq_fused = fused_pq ? wht_ydot_partials(g0) : 0;
+                        wht_partials(g0, sizeof(double), TM_YDOT) <= kRedBlocks;
+  const int n_pq_fused = fused_pq ? wht_partials(g0, sizeof(double), TM_YDOT) : 0;
   // FP32 V-cycles: the last fine-grid sweep (same operator) yields r.z, with r
   // as the V-cycle's FP32 copy (the FP64 sweep has no registers to spare)
   const bool fused_rz = fused_pq && sizeof(T) == sizeof(float) && get_fused_rz() &&
-                        wht_supported(g0, sizeof(T)) && !sh.mfma && h.n_smooth > 0;
-  const int n_rz_fused = fused_rz ? wht_ydot_partials(g0) : kRedBlocks;
+                        wht_supported(g0, sizeof(T)) && wht_partials(g0, sizeof(T), TM_JAC_ADD_DOT) <= kRedBlocks &&
+                        !sh.mfma && h.n_smooth > 0;
+  const int n_rz_fused = fused_rz ? wht_partials(g0, sizeof(T), TM_JAC_ADD_DOT) : kRedBlocks;
   int last = 0;
   auto read_ring = [&](int k, double* R, int* st) -> bool {
     ET_CHECK_HIP(hipEventSynchronize(ring_ev[k & 1]));

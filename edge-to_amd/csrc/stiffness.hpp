@@ -2,6 +2,7 @@
 // solver and the C-ABI layer.
 #pragma once
 #include "common.hpp"
+#include "op_tiled.hpp"
 
 namespace et {
 
@@ -38,13 +39,14 @@ int op0_wht(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, co
             double wscale, const T* xadd, T* y, hipStream_t s);
 bool wht_supported(const Grid& g, size_t elem_bytes);
 // y = A x (FP64) fused with the x.y partial sum of every workgroup (fixed order);
-// writes wht_ydot_partials(g) <= max_partial values to partial.
+// writes wht_partials(g, sizeof(T), TM_YDOT) <= max_partial values to partial.
 template <typename T>
 int op0_wht_ydot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E, const uint8_t* cons,
                  const T* x, T* y, double* partial, int max_partial, hipStream_t s);
-int wht_ydot_partials(const Grid& g);
+// number of per-workgroup partial sums written by the fused-dot variants
+int wht_partials(const Grid& g, size_t elem_bytes, int mode);
 // Final post-smoothing sweep y = xadd + Jacobi(x; b) with the per-workgroup
-// partial sums of dotv.y (same partial count as op0_wht_ydot).
+// partial sums of dotv.y (wht_partials(g, sizeof(T), TM_JAC_ADD_DOT) values).
 template <typename T>
 int op0_wht_jac_add_dot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E, const uint8_t* cons,
                         const T* x, const T* b, const T* Dinv, const double* omega, double wscale, const T* xadd,
@@ -55,6 +57,8 @@ void set_wht_wgs(int n);
 int get_wht_wgs();
 void set_wht_threads(int is_double, int nt);
 int get_wht_threads(int is_double);
+void set_wht_slices(int is_double, int ns);
+int get_wht_slices(int is_double);
 // Fine-grid dispatch: the transformed-basis operator where enabled and
 // supported, else the matrix-core one where enabled, else the LDS-tiled one.
 template <typename T>

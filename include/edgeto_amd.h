@@ -58,13 +58,19 @@ const char* et_version(void);
  *   matrix-core operator (persistent grid).
  * ET_TUNE_WHT: 1 (default) = the fine-grid operator (both precisions) runs in the corner
  *   Walsh-Hadamard basis (45 FMAs per element); takes precedence over ET_TUNE_MX_*.
- * ET_TUNE_WHT_WGS: workgroups per compute unit launched by that operator.
+ * ET_TUNE_WHT_WGS: workgroups per compute unit launched by that operator for 16-slice
+ *   tiles (8-slice tiles launch twice as many).
  * ET_TUNE_COARSE_LIB: factor path of the coarsest grid, read by et_mg_create:
  *   2 (default) = band Cholesky (in-house tile kernels, rocBLAS band updates) and
  *   L^-1 by rocSOLVER trtri, coarse solve by two triangular products; 1 = rocSOLVER
  *   potrf + potri; 0 = in-house dense blocked Cholesky and inverse kernels.
- * ET_TUNE_WHT_THREADS_F32 / _F64: workgroup size (512 or 1024) of the transformed-basis
- *   operator.
+ * ET_TUNE_WHT_THREADS_F32 / _F64: workgroup size of the transformed-basis operator:
+ *   1024 or 512 for 16-slice tiles, 512 or 256 for 8-slice tiles (other values map to
+ *   the nearer of the two).
+ * ET_TUNE_WHT_SLICES_F32 / _F64: 64-element z slices per layer of that operator's x tile
+ *   in its residual and Jacobi modes, 16 (8 element columns for nelz <= 128) or 8 (half
+ *   the tile: fewer registers per thread, more halo recomputation); defaults 16 (FP32)
+ *   and 8 (FP64).  y = A x products always use 16-slice tiles.
  * ET_TUNE_ROW_STENCIL_NODES: coarse levels with at most this many nodes apply their
  *   stencil with one thread per (node, component) instead of one per node.
  * ET_TUNE_LEVEL1_MF: 1 (default) = the first coarse level of 3-D hierarchies with even
@@ -89,7 +95,9 @@ enum {
   ET_TUNE_WHT_THREADS_F64 = 11,
   ET_TUNE_ROW_STENCIL_NODES = 12,
   ET_TUNE_LEVEL1_MF = 13,
-  ET_TUNE_FUSED_RZ = 14
+  ET_TUNE_FUSED_RZ = 14,
+  ET_TUNE_WHT_SLICES_F32 = 15,
+  ET_TUNE_WHT_SLICES_F64 = 16
 };
 int et_tune_set(int param, int value);
 int et_tune_get(int param);

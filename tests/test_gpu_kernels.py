@@ -314,10 +314,12 @@ MX_MESHES = [(6, 4, 4), (8, 6, 20), (5, 3, 33), (12, 5, 16), (20, 3, 128), (9, 1
 WHT_MESHES = MX_MESHES + [(7, 4, 64), (5, 3, 65), (30, 4, 100), (4, 2, 1)]
 
 # fine-operator variants by knob values
-# (ET_TUNE_WHT, ET_TUNE_MX_F32, ET_TUNE_MX_F64, ET_TUNE_WHT_THREADS_F32, ET_TUNE_WHT_THREADS_F64)
-_KN = (7, 3, 4, 10, 11)
-_FINE = {"tiled": (0, 0, 0, 1024, 512), "matrix": (0, 1, 1, 1024, 512), "wht": (1, 0, 0, 1024, 512),
-         "wht_alt": (1, 0, 0, 512, 1024)}
+# (ET_TUNE_WHT, ET_TUNE_MX_F32, ET_TUNE_MX_F64, ET_TUNE_WHT_THREADS_F32, ET_TUNE_WHT_THREADS_F64,
+#  ET_TUNE_WHT_SLICES_F32, ET_TUNE_WHT_SLICES_F64)
+_KN = (7, 3, 4, 10, 11, 15, 16)
+_FINE = {"tiled": (0, 0, 0, 1024, 512, 16, 8), "matrix": (0, 1, 1, 1024, 512, 16, 8),
+         "wht": (1, 0, 0, 1024, 512, 16, 8), "wht_alt": (1, 0, 0, 512, 1024, 16, 16),
+         "wht_half": (1, 0, 0, 512, 256, 8, 8)}
 
 
 def _matvec_variant(L, nel, prec, variant, seed):
@@ -350,9 +352,9 @@ def test_matvec_matrix_core(nel, prec):
 
 @pytest.mark.parametrize("nel", WHT_MESHES)
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-@pytest.mark.parametrize("variant", ["wht", "wht_alt"])
+@pytest.mark.parametrize("variant", ["wht", "wht_alt", "wht_half"])
 def test_matvec_transformed_basis(nel, prec, variant):
-    """Walsh-Hadamard-basis fine operator (both workgroup sizes): both tile widths (nelz <= 64
+    """Walsh-Hadamard-basis fine operator (every workgroup size): both tile widths (nelz <= 64
     and <= 128), the 64 and 128 seams, a single z layer, and more (x tile, plane) units than
     persistent workgroups."""
     y, ref = _matvec_variant(_lib(), nel, prec, variant, 22)
@@ -364,10 +366,11 @@ SWEEP_MESHES = [(6, 4, 4), (7, 5), (20, 3, 128), (48, 20, 100), (40, 30, 40)]
 
 @pytest.mark.parametrize("nel", SWEEP_MESHES)
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-@pytest.mark.parametrize("variant", ["tiled", "matrix", "wht", "wht_alt"])
+@pytest.mark.parametrize("variant", ["tiled", "matrix", "wht", "wht_alt", "wht_half"])
 def test_residual_and_jacobi(nel, prec, variant):
     """Fused residual and damped-Jacobi modes of every fine-grid operator variant against the
-    oracle (r = b - A x, y = x + w D^-1 r), on meshes with many y planes per workgroup."""
+    oracle (r = b - A x, y = x + w D^-1 r), on meshes with many y planes per workgroup; the
+    transformed-basis sweeps on 16- and 8-slice tiles with every workgroup size."""
     L = _lib()
     dim = len(nel)
     old = [L.lib().et_tune_get(k) for k in _KN]
