@@ -304,9 +304,10 @@ __global__ __launch_bounds__(256) void lower_transpose_kernel(const double* __re
 template <typename TI>
 __global__ __launch_bounds__(256) void trmv_lower_kernel(const double* __restrict__ Z, int ld, int n,
                                                          const TI* __restrict__ b, double* __restrict__ t) {
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // longest rows first (workgroups start roughly in index order): a shorter tail
+  const int i = n - 1 - (blockIdx.x * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
-  if (i >= n) return;
+  if (i < 0) return;
   const double* z = Z + (int64_t)i * ld;
   // four independent partial sums: four loads of each operand in flight per lane
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;

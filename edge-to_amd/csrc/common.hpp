@@ -120,4 +120,20 @@ __device__ inline T block_sum256_all(T v) {
   return tot;
 }
 
+// Global accesses at a 32-bit element index from a kernel-argument base: the
+// byte offset is formed in 32 bits and zero-extended, so the access takes the
+// scalar-base + vector-offset form with no 64-bit address arithmetic per access
+// (the host checks that every byte offset fits in 32 bits).
+// Element i + k: the constant k goes into the instruction's offset field.
+template <typename T>
+__device__ inline T ld32(const T* __restrict__ p, int i, int k = 0) {
+  const char* a = reinterpret_cast<const char*>(p) + (size_t)((unsigned)i * (unsigned)sizeof(T));
+  return *reinterpret_cast<const T*>(a + (size_t)k * sizeof(T));
+}
+template <typename T>
+__device__ inline void st32(T* __restrict__ p, int i, int k, T v) {
+  char* a = reinterpret_cast<char*>(p) + (size_t)((unsigned)i * (unsigned)sizeof(T));
+  *reinterpret_cast<T*>(a + (size_t)k * sizeof(T)) = v;
+}
+
 }  // namespace et
