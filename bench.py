@@ -219,7 +219,7 @@ def roofline(prof, n_dof, n_el, workload):
         "l0_jacobi": s * (4 * n_dof + n_el) + mask,       # read x, b, D^-1, E; write y
         "l0_jacobi0": s * (3 * n_dof + n_el) + mask,      # read b, D^-1, E; write y (x0 = w D^-1 b)
         "l0_residual": s * (3 * n_dof + n_el) + mask,     # read x, b, E; write r
-        "l0_jacobi_add": s * (5 * n_dof + n_el) + mask,   # + read x_prev
+        "l0_jacobi_add": s * (6 * n_dof + n_el) + mask,   # + read x_prev and r (FP32 copy, for r.z)
         "matvec64": 8 * (2 * n_dof + n_el) + mask,        # FP64: read x, E; write y
     }
     cand = {k: v for k, v in prof.items() if k in bytes_per and v[1] > 0}
