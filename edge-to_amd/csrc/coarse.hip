@@ -240,13 +240,14 @@ __global__ __launch_bounds__(256) void yty_kernel(const double* Y, double* Ainv,
 // b = dof (nx nz + nz + 1) + dof - 1, so tile (i, m) of A and of L is zero
 // for i - m > nbd = ceil(b / 64), and each factor step k touches only the nr
 // = min(nbd, tiles left) tiles below the diagonal:
-//   diag:   one wavefront factors A_kk and forms W_kk = L_kk^-1 (chol_tile.hpp);
+//   diag:   one workgroup (four wavefronts) factors A_kk and forms W_kk = L_kk^-1 (chol_tile.hpp);
 //   panel:  A_ik <- A_ik W_kk^T for the nr tiles below;
-//   update: A_22 -= L_21 L_21^T on the nr x nr band block (rocBLAS dsyrk).
+//   update: A_22 -= L_21 L_21^T on the nr x nr band block (one workgroup per
+//           lower tile: one launch where rocBLAS dsyrk took five).
 // rocSOLVER trtri then overwrites L with Y = L^-1, and the coarse solve is
 // x = Y^T (Y b): two triangular products reading n^2 / 2 values each.
 
-__global__ __launch_bounds__(64) void bchol_diag_kernel(double* A, int ld, int o, double* Wk, int* info) {
+__global__ __launch_bounds__(kCholTileThreads) void bchol_diag_kernel(double* A, int ld, int o, double* Wk, int* info) {
   chol_tile64(A, ld, o, Wk, info);
 }
 
@@ -279,6 +280,44 @@ __global__ __launch_bounds__(256) void bchol_panel_kernel(double* A, int ld, int
   for (int q = 0; q < 4; ++q)
 #pragma unroll
     for (int p = 0; p < 4; ++p) A[(int64_t)(o + c0 + q) * ld + row0 + r0 + p] = acc[p][q];
+}
+
+// update: A_22 -= L_21 L_21^T on the lower tiles of the nr x nr band block below
+// the diagonal tile kb (L_21: the nr panel tiles).  One workgroup per lower tile
+// (i, j), j <= i; each sums over the 64 panel columns in order.  Upper halves of
+// the diagonal tiles are left as they are (never read as L).
+__global__ __launch_bounds__(256) void bchol_syrk_kernel(double* A, int ld, int kb) {
+  __shared__ double Xs[NB][NB + 1];  // L_i [row][m]
+  __shared__ double Ys[NB][NB + 1];  // L_j [row][m]
+  int i = 0;
+  while ((i + 1) * (i + 2) / 2 <= (int)blockIdx.x) ++i;
+  const int j = (int)blockIdx.x - i * (i + 1) / 2;
+  const int o = kb * NB, ri = o + NB * (1 + i), rj = o + NB * (1 + j);
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    const int rr = t % NB, cc = t / NB;  // consecutive threads, consecutive rows (column-major)
+    Xs[rr][cc] = A[(int64_t)(o + cc) * ld + ri + rr];
+    Ys[rr][cc] = A[(int64_t)(o + cc) * ld + rj + rr];
+  }
+  __syncthreads();
+  const int r0 = (threadIdx.x % 16) * 4, c0 = (threadIdx.x / 16) * 4;
+  double acc[4][4] = {};
+  for (int m = 0; m < NB; ++m) {
+    double xa[4], yb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xa[u] = Xs[r0 + u][m];
+      yb[u] = Ys[c0 + u][m];
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[p][q] += xa[p] * yb[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      if (i > j || r0 + p >= c0 + q) A[(int64_t)(rj + c0 + q) * ld + ri + r0 + p] -= acc[p][q];
 }
 
 // Z = Y row-major (Z[i ld + j] = Y_ij) for the tiles on and below the diagonal
@@ -436,19 +475,15 @@ int CoarseSolver::factor(const Grid& g, int dof, const double* S, hipStream_t s)
     const int64_t bw = g.dim == 3 ? (int64_t)dof * ((int64_t)g.nx() * g.nz() + g.nz() + 1) + dof - 1
                                   : (int64_t)dof * (g.nx() + 1) + dof - 1;
     const int nbd = (int)std::min<int64_t>((bw + NB - 1) / NB, nbt - 1);
-    const double minus1 = -1.0, one = 1.0;
     for (int kb = 0; kb < nbt; ++kb) {
       const int nr = std::min(nbd, nbt - kb - 1);
-      bchol_diag_kernel<<<1, 64, 0, s>>>(A, ld, kb * NB, Wd, info);
+      bchol_diag_kernel<<<1, kCholTileThreads, 0, s>>>(A, ld, kb * NB, Wd, info);
       ET_CHECK_LAUNCH();
       if (nr > 0) {
         bchol_panel_kernel<<<nr, 256, 0, s>>>(A, ld, kb, Wd);
         ET_CHECK_LAUNCH();
-        const int o = kb * NB, m = nr * NB;
-        ET_REQUIRE(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, m, NB, &minus1,
-                                 A + (int64_t)o * ld + o + NB, ld, &one, A + (int64_t)(o + NB) * ld + o + NB,
-                                 ld) == rocblas_status_success,
-                   "rocblas_dsyrk failed");
+        bchol_syrk_kernel<<<nr * (nr + 1) / 2, 256, 0, s>>>(A, ld, kb);
+        ET_CHECK_LAUNCH();
       }
     }
     ET_REQUIRE(rocsolver_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_non_unit, ld, A, ld, info + 1) ==
