@@ -127,50 +127,78 @@ template int restrict_op<float>(const Grid&, int, const float*, float*, hipStrea
 // One thread per fine node: average of the (1, 2, 4 or 8) in-range coarse
 // nodes of its cell, each divided by its total weight tw.
 
-// Branch-free: the cell's up to 8 coarse corners are read at clamped indices;
-// an axis contributes its second corner (weight 1) only when the fine node is
-// odd on it and that corner exists.
+// Prolongation P = P_lin D_tw^-1, one thread per coarse node (ci, cj, ck): it
+// writes the fine nodes (2ci + a, 2cj + b, 2ck + c), a, b, c in {0, 1}, whose
+// parents all lie in {ci, ci+1} x {cj, cj+1} x {ck, ck+1}, so the 8 scaled
+// parent values are loaded once per thread (not once per fine node).  A fine
+// value is the sum of its parents' scaled values, in j, i, k order, times
+// 1 / (number of parents); parents past the upper boundary are read clamped
+// and not used.
 template <typename T, int DIM, int DOF>
 __global__ __launch_bounds__(256) void prolong_kernel(const Grid gf, const Grid gc,
                                                       const T* __restrict__ vc,
                                                       T* __restrict__ vf) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= (int)gf.n_nodes()) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int)gc.n_nodes()) return;
   const int cnx = gc.nx(), cny = gc.ny(), cnz = gc.nz();
-  int i, j, k;
-  node_coords32<DIM>(f, gf.nx(), gf.nz(), i, j, k);
-  const int ci0 = i >> 1, cj0 = j >> 1, ck0 = k >> 1;
-  const bool two_i = (i & 1) && ci0 + 1 < cnx;
-  const bool two_j = (j & 1) && cj0 + 1 < cny;
-  const bool two_k = DIM == 3 && (k & 1) && ck0 + 1 < cnz;
-  const int ci1 = two_i ? ci0 + 1 : ci0, cj1 = two_j ? cj0 + 1 : cj0, ck1 = two_k ? ck0 + 1 : ck0;
-  T acc[DOF];
+  const int fnx = gf.nx(), fny = gf.ny(), fnz = gf.nz();
+  int ci, cj, ck;
+  node_coords32<DIM>(c, cnx, cnz, ci, cj, ck);
+  constexpr int KZ = DIM == 3 ? 2 : 1;
+  T sv[2][2][KZ][DOF];
 #pragma unroll
-  for (int d = 0; d < DOF; ++d) acc[d] = T(0);
+  for (int dj = 0; dj < 2; ++dj)
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+    for (int di = 0; di < 2; ++di)
 #pragma unroll
-    for (int bq = 0; bq < 2; ++bq)
+      for (int dk = 0; dk < KZ; ++dk) {
+        const int i = min(ci + di, cnx - 1), j = min(cj + dj, cny - 1), k = DIM == 3 ? min(ck + dk, cnz - 1) : 0;
+        const T inv = inv_tw_of<T, DIM>(gc, i, j, k);
+        const int n = DIM == 3 ? (j * cnx + i) * cnz + k : j * cnx + i;
 #pragma unroll
-      for (int cq = 0; cq < (DIM == 3 ? 2 : 1); ++cq) {
-        const bool use = (a == 0 || two_j) && (bq == 0 || two_i) && (cq == 0 || two_k);
-        const int ci = bq ? ci1 : ci0, cj = a ? cj1 : cj0, ck = cq ? ck1 : ck0;
-        const T inv = use ? inv_tw_of<T, DIM>(gc, ci, cj, ck) : T(0);
-        const int c = DIM == 3 ? (cj * cnx + ci) * cnz + ck : cj * cnx + ci;
-#pragma unroll
-        for (int d = 0; d < DOF; ++d) acc[d] += vc[c * DOF + d] * inv;
+        for (int d = 0; d < DOF; ++d) sv[dj][di][dk][d] = vc[n * DOF + d] * inv;
       }
-  const int count = (two_i ? 2 : 1) * (two_j ? 2 : 1) * (two_k ? 2 : 1);
-  const T sc = T(1) / (T)count;
 #pragma unroll
-  for (int d = 0; d < DOF; ++d) vf[f * DOF + d] = acc[d] * sc;
+  for (int b = 0; b < 2; ++b) {
+    const int fj = 2 * cj + b;
+    const bool two_j = b == 1 && cj + 1 < cny;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int fi = 2 * ci + a;
+      const bool two_i = a == 1 && ci + 1 < cnx;
+#pragma unroll
+      for (int cc = 0; cc < KZ; ++cc) {
+        const int fk = 2 * ck + cc;
+        const bool two_k = cc == 1 && ck + 1 < cnz;
+        if (fj >= fny || fi >= fnx || (DIM == 3 && fk >= fnz)) continue;
+        T acc[DOF];
+#pragma unroll
+        for (int d = 0; d < DOF; ++d) acc[d] = T(0);
+#pragma unroll
+        for (int pj = 0; pj < 2; ++pj)
+#pragma unroll
+          for (int pi = 0; pi < 2; ++pi)
+#pragma unroll
+            for (int pk = 0; pk < KZ; ++pk) {
+              if ((pj == 1 && !two_j) || (pi == 1 && !two_i) || (pk == 1 && !two_k)) continue;
+#pragma unroll
+              for (int d = 0; d < DOF; ++d) acc[d] += sv[pj][pi][pk][d];
+            }
+        const int count = (two_i ? 2 : 1) * (two_j ? 2 : 1) * (two_k ? 2 : 1);
+        const T scl = T(1) / (T)count;
+        const int f = DIM == 3 ? (fj * fnx + fi) * fnz + fk : fj * fnx + fi;
+#pragma unroll
+        for (int d = 0; d < DOF; ++d) vf[f * DOF + d] = acc[d] * scl;
+      }
+    }
+  }
 }
 
 template <typename T>
 int prolong_op(const Grid& gf, int dof, const T* vc, T* vf, hipStream_t s) {
   ET_REQUIRE(dof >= 1 && dof <= 3, "prolongation: dof must be 1, 2 or 3");
   const Grid gc = gf.coarse();
-  const unsigned grid = grid_for(gf.n_nodes());
+  const unsigned grid = grid_for(gc.n_nodes());
 #define ET_P(DIM, DOF) prolong_kernel<T, DIM, DOF><<<grid, kBlock, 0, s>>>(gf, gc, vc, vf)
   if (gf.dim == 3) {
     if (dof == 3) ET_P(3, 3); else if (dof == 2) ET_P(3, 2); else ET_P(3, 1);
