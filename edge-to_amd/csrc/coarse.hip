@@ -343,21 +343,25 @@ __global__ __launch_bounds__(256) void lower_transpose_kernel(const double* __re
 template <typename TI>
 __global__ __launch_bounds__(256) void trmv_lower_kernel(const double* __restrict__ Z, int ld, int n,
                                                          const TI* __restrict__ b, double* __restrict__ t) {
-  // longest rows first (workgroups start roughly in index order): a shorter tail
   const int i = n - 1 - (blockIdx.x * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (i < 0) return;
   const double* z = Z + (int64_t)i * ld;
-  // four independent partial sums: four loads of each operand in flight per lane
+  // pairs of consecutive values per lane (16-byte loads; rows start 16-byte aligned, ld even)
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int j = lane;
-  for (; j + 192 <= i; j += 256) {
-    s0 += z[j] * (double)b[j];
-    s1 += z[j + 64] * (double)b[j + 64];
-    s2 += z[j + 128] * (double)b[j + 128];
-    s3 += z[j + 192] * (double)b[j + 192];
+  int j = 2 * lane;
+  for (; j + 257 <= i; j += 256) {
+    const double2 za = *reinterpret_cast<const double2*>(z + j);
+    const double2 zb = *reinterpret_cast<const double2*>(z + j + 128);
+    s0 += za.x * (double)b[j];
+    s1 += za.y * (double)b[j + 1];
+    s2 += zb.x * (double)b[j + 128];
+    s3 += zb.y * (double)b[j + 129];
   }
-  for (; j <= i; j += 64) s0 += z[j] * (double)b[j];
+  for (; j <= i; j += 128) {
+    s0 += z[j] * (double)b[j];
+    if (j + 1 <= i) s1 += z[j + 1] * (double)b[j + 1];
+  }
   const double s = wave_sum((s0 + s1) + (s2 + s3));
   if (lane == 0) t[i] = s;
 }
@@ -371,14 +375,20 @@ __global__ __launch_bounds__(256) void trmv_upper_kernel(const double* __restric
   if (j >= n) return;
   const double* y = Y + (int64_t)j * ld;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int i = j + lane;
-  for (; i + 192 < n; i += 256) {
-    s0 += y[i] * t[i];
-    s1 += y[i + 64] * t[i + 64];
-    s2 += y[i + 128] * t[i + 128];
-    s3 += y[i + 192] * t[i + 192];
+  // start at the even index at or below j; the entry above the diagonal (i = j - 1) is zero
+  int i = (j & ~1) + 2 * lane;
+  for (; i + 129 < n; i += 256) {
+    const double2 ya = *reinterpret_cast<const double2*>(y + i);
+    const double2 yb = *reinterpret_cast<const double2*>(y + i + 128);
+    s0 += ya.x * t[i];
+    s1 += ya.y * t[i + 1];
+    s2 += yb.x * t[i + 128];
+    s3 += yb.y * t[i + 129];
   }
-  for (; i < n; i += 64) s0 += y[i] * t[i];
+  for (; i < n; i += 128) {
+    s0 += y[i] * t[i];
+    if (i + 1 < n) s1 += y[i + 1] * t[i + 1];
+  }
   const double s = wave_sum((s0 + s1) + (s2 + s3));
   if (lane == 0) x[j] = (TO)s;
 }

@@ -208,6 +208,11 @@ __global__ __launch_bounds__(256) void l1_elem_kernel(const Grid g1, const T* __
   const int lane = threadIdx.x & 63;
   const int K = C % g1.ez, IJ = C / g1.ez, J = IJ % g1.ey, I = IJ / g1.ey;
   const int nx = g1.nx(), nz = g1.nz();
+  // the element's transformed densities are requested before the corner values
+  // are waited on (they are needed only after the forward transforms)
+  T eh[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) eh[m] = Eh[(int64_t)C * 8 + m];
   T xh[3][8];
 #pragma unroll
   for (int beta = 0; beta < 8; ++beta) {
@@ -219,9 +224,6 @@ __global__ __launch_bounds__(256) void l1_elem_kernel(const Grid g1, const T* __
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) l1_wht8(xh[c]);
-  T eh[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) eh[m] = Eh[(int64_t)C * 8 + m];
   T yh[3][8];
   // the coefficients are wave-uniform: scalar loads into SGPRs (staging them
   // in LDS lets the compiler hoist every read and quadruples the VGPRs)
