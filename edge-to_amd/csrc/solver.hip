@@ -204,8 +204,8 @@ __global__ __launch_bounds__(256) void opS_row_kernel(const Grid g, const T* __r
       const int m = in ? n + off : n;
 #pragma unroll
       for (int q = 0; q < DIM; ++q) {
-        const T xv = in ? x[m * DIM + q] : T(0);
-        acc += S[((h * DIM + r) * DIM + q) * N + n] * xv;
+        const T xv = in ? ld32(x, m * DIM, q) : T(0);
+        acc += ld32(S, ((h * DIM + r) * DIM + q) * N + n) * xv;
       }
     }
     if (h > 0) {
@@ -214,22 +214,22 @@ __global__ __launch_bounds__(256) void opS_row_kernel(const Grid g, const T* __r
       const int m = in ? n - off : n;
 #pragma unroll
       for (int q = 0; q < DIM; ++q) {
-        const T xv = in ? x[m * DIM + q] : T(0);
-        acc += S[((h * DIM + q) * DIM + r) * N + m] * xv;
+        const T xv = in ? ld32(x, m * DIM, q) : T(0);
+        acc += ld32(S, ((h * DIM + q) * DIM + r) * N + m) * xv;
       }
     }
   }
-  const int64_t d = (int64_t)n * DIM + r;
+  const int d = n * DIM + r;
   const T w = MODE >= OP_JAC ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
   T v;
   if (MODE == OP_Y)
     v = acc;
   else if (MODE == OP_RES)
-    v = b[d] - acc;
+    v = ld32(b, d) - acc;
   else
-    v = x[d] + w * Dinv[d] * (b[d] - acc);
-  if (MODE == OP_JAC_ADD) v += xadd[d];
-  y[d] = v;
+    v = ld32(x, d) + w * ld32(Dinv, d) * (ld32(b, d) - acc);
+  if (MODE == OP_JAC_ADD) v += ld32(xadd, d);
+  st32(y, d, 0, v);
 }
 
 // coarse levels up to this size use opS_row_kernel (32-bit stencil offsets: also below 2^31 / 126 nodes)
@@ -460,7 +460,7 @@ static int op_launch(const MGLevel<T>& L, const MGShared& sh, int mode, const T*
                                                    L.Dinv, omega, wscale, xadd, y)
 #define ET_OPS(DIM, MODE)                                                                            \
   do {                                                                                               \
-    if (g.n_nodes() <= g_row_stencil_max_nodes && g.n_nodes() * 126 < ((int64_t)1 << 31))          \
+    if (g.n_nodes() <= g_row_stencil_max_nodes && g.n_nodes() * 126 * (int64_t)sizeof(T) < ((int64_t)1 << 32)) \
       opS_row_kernel<T, DIM, MODE><<<grid_for(g.n_nodes() * DIM), kBlock, 0, s>>>(g, L.S, x, b, L.Dinv, \
                                                                                   omega, wscale, xadd, y); \
     else                                                                                             \
