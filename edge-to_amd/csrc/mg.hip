@@ -66,10 +66,15 @@ __device__ __forceinline__ void node_coords32(int n, int nx, int nz, int& i, int
 
 // Branch-free: out-of-range neighbours read a clamped (in-range) node with
 // weight 0, so all loads of a thread are in flight together.
+// With x0 given, also x0 = w D^-1 vc, w = omega * wscale (the coarse level's first
+// Jacobi sweep from the zero guess, rounded as scale_kernel rounds it).
 template <typename T, int DIM, int DOF>
 __global__ __launch_bounds__(256) void restrict_kernel(const Grid gf, const Grid gc,
                                                        const T* __restrict__ vf,
-                                                       T* __restrict__ vc) {
+                                                       T* __restrict__ vc,
+                                                       const T* __restrict__ Dinv,
+                                                       const double* __restrict__ omega, double wscale,
+                                                       T* __restrict__ x0) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= (int)gc.n_nodes()) return;
   const int fnx = gf.nx(), fny = gf.ny(), fnz = gf.nz();
@@ -103,14 +108,22 @@ __global__ __launch_bounds__(256) void restrict_kernel(const Grid gf, const Grid
   const T inv = inv_tw_of<T, DIM>(gc, ci, cj, ck);
 #pragma unroll
   for (int d = 0; d < DOF; ++d) vc[c * DOF + d] = acc[d] * inv;
+  if (x0 != nullptr) {
+    const T w = (T)(*omega * wscale);
+#pragma unroll
+    for (int d = 0; d < DOF; ++d) x0[c * DOF + d] = w * Dinv[c * DOF + d] * (acc[d] * inv);
+  }
 }
 
 template <typename T>
-int restrict_op(const Grid& gf, int dof, const T* vf, T* vc, hipStream_t s) {
+int restrict_op(const Grid& gf, int dof, const T* vf, T* vc, hipStream_t s, const T* Dinv,
+                const double* omega, double wscale, T* x0) {
   ET_REQUIRE(dof >= 1 && dof <= 3, "restriction: dof must be 1, 2 or 3");
+  ET_REQUIRE(x0 == nullptr || (Dinv != nullptr && omega != nullptr), "restriction: x0 needs Dinv and omega");
   const Grid gc = gf.coarse();
   const unsigned grid = grid_for(gc.n_nodes());
-#define ET_R(DIM, DOF) restrict_kernel<T, DIM, DOF><<<grid, kBlock, 0, s>>>(gf, gc, vf, vc)
+#define ET_R(DIM, DOF) \
+  restrict_kernel<T, DIM, DOF><<<grid, kBlock, 0, s>>>(gf, gc, vf, vc, Dinv, omega, wscale, x0)
   if (gf.dim == 3) {
     if (dof == 3) ET_R(3, 3); else if (dof == 2) ET_R(3, 2); else ET_R(3, 1);
   } else {
@@ -120,8 +133,10 @@ int restrict_op(const Grid& gf, int dof, const T* vf, T* vc, hipStream_t s) {
   ET_CHECK_LAUNCH();
   return ET_OK;
 }
-template int restrict_op<double>(const Grid&, int, const double*, double*, hipStream_t);
-template int restrict_op<float>(const Grid&, int, const float*, float*, hipStream_t);
+template int restrict_op<double>(const Grid&, int, const double*, double*, hipStream_t, const double*,
+                                  const double*, double, double*);
+template int restrict_op<float>(const Grid&, int, const float*, float*, hipStream_t, const float*,
+                                 const double*, double, float*);
 
 // ----------------------------------------------------------- prolongation ---
 // One thread per fine node: average of the (1, 2, 4 or 8) in-range coarse

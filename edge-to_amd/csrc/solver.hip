@@ -495,7 +495,7 @@ static int op_launch(const MGLevel<T>& L, const MGShared& sh, int mode, const T*
 }
 
 template <typename T>
-int MGHierarchy<T>::vcycle(int l, MGShared& sh, hipStream_t s) {
+int MGHierarchy<T>::vcycle(int l, MGShared& sh, hipStream_t s, bool x0_ready) {
   MGLevel<T>& L = lv[l];
   if (l == n_level) {
     sh.prof->begin(ET_PROF_COARSE_SOLVE, s);
@@ -516,7 +516,7 @@ int MGHierarchy<T>::vcycle(int l, MGShared& sh, hipStream_t s) {
       ET_TRY(op_launch<T>(L, sh, OP_JAC0, nullptr, L.b, om, ws, nullptr, L.t, s));
       std::swap(L.x, L.t);
       it = 2;
-    } else {
+    } else if (!x0_ready) {
       sh.prof->begin(l == 0 ? ET_PROF_L0_SCALE : ET_PROF_COARSE_OPS, s);
       scale_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, L.Dinv, L.b, om, ws, L.x);
       sh.prof->end(s);
@@ -531,9 +531,13 @@ int MGHierarchy<T>::vcycle(int l, MGShared& sh, hipStream_t s) {
   for (int pass = 0; pass < passes; ++pass) {
     ET_TRY(op_launch<T>(L, sh, OP_RES, L.x, L.b, om, ws, nullptr, L.r, s));
     sh.prof->begin(ET_PROF_TRANSFER, s);
-    ET_TRY(restrict_op<T>(L.g, dof, L.r, lv[l + 1].b, s));
+    // a smoothed coarse level starts from x = w D^-1 b: written by the restriction
+    const bool x0 = l + 1 < n_level && n_smooth > 0;
+    MGLevel<T>& C = lv[l + 1];
+    ET_TRY(restrict_op<T>(L.g, dof, L.r, C.b, s, x0 ? C.Dinv : nullptr, om, std::pow(1.06, l + 1),
+                          x0 ? C.x : nullptr));
     sh.prof->end(s);
-    ET_TRY(vcycle(l + 1, sh, s));
+    ET_TRY(vcycle(l + 1, sh, s, x0));
     sh.prof->begin(ET_PROF_TRANSFER, s);
     ET_TRY(prolong_op<T>(L.g, dof, lv[l + 1].x, L.e, s));
     sh.prof->end(s);

@@ -108,7 +108,8 @@ struct MGHierarchy {
   // skip_s1: level 1 is applied matrix-free (no stencil copy needed)
   int alloc(const Grid& g0, int dof, int n_level, hipStream_t s, bool skip_s1 = false);
   void release();
-  int vcycle(int l, MGShared& sh, hipStream_t s);
+  // x0_ready: level l's x already holds w D^-1 b (written by the restriction)
+  int vcycle(int l, MGShared& sh, hipStream_t s, bool x0_ready = false);
 };
 
 struct MGSolver {

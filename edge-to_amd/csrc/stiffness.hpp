@@ -105,7 +105,9 @@ int filter_adj(const Grid& g, int n_off, const int32_t* off, const double* w, co
 
 // transfers (g = fine grid)
 template <typename T>
-int restrict_op(const Grid& g, int dof, const T* vf, T* vc, hipStream_t s);
+// with x0: also x0 = omega * wscale * Dinv * vc (first Jacobi sweep from zero on the coarse grid)
+int restrict_op(const Grid& g, int dof, const T* vf, T* vc, hipStream_t s, const T* Dinv = nullptr,
+                const double* omega = nullptr, double wscale = 1.0, T* x0 = nullptr);
 template <typename T>
 int prolong_op(const Grid& g, int dof, const T* vc, T* vf, hipStream_t s);
 
