@@ -510,8 +510,8 @@ int MGHierarchy<T>::vcycle(int l, MGShared& sh, hipStream_t s, bool x0_ready) {
     ET_CHECK_HIP(hipMemsetAsync(L.x, 0, n * sizeof(T), s));
   } else {
     int it = 1;
-    // FP32 only: the FP64 operator is register-bound and runs the two passes faster
-    if (l == 0 && sizeof(T) == 4 && n_smooth >= 2 && sh.tiled && !sh.mfma && fine_jac0_supported(L.g, sizeof(T))) {
+    // both precisions (the FP64 sweeps run on 8-slice tiles, where the fused pass is faster)
+    if (l == 0 && n_smooth >= 2 && sh.tiled && !(sizeof(T) == 4 && sh.mfma) && fine_jac0_supported(L.g, sizeof(T))) {
       // x0 = w D^-1 b and the first sweep from it in one pass
       ET_TRY(op_launch<T>(L, sh, OP_JAC0, nullptr, L.b, om, ws, nullptr, L.t, s));
       std::swap(L.x, L.t);
@@ -935,9 +935,9 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
   const bool fused_pq = sh.tiled && g0.dim == 3 && get_wht() && wht_supported(g0, sizeof(double)) &&
                         wht_partials(g0, sizeof(double), TM_YDOT) <= kRedBlocks;
   const int n_pq_fused = fused_pq ? wht_partials(g0, sizeof(double), TM_YDOT) : 0;
-  // FP32 V-cycles: the last fine-grid sweep (same operator) yields r.z, with r
-  // as the V-cycle's FP32 copy (the FP64 sweep has no registers to spare)
-  const bool fused_rz = fused_pq && sizeof(T) == sizeof(float) && get_fused_rz() &&
+  // the last fine-grid sweep of the V-cycle (same operator) yields r.z, with r
+  // in the V-cycle's precision (FP32 V-cycles: the copy of r they read)
+  const bool fused_rz = fused_pq && get_fused_rz() &&
                         wht_supported(g0, sizeof(T)) && wht_partials(g0, sizeof(T), TM_JAC_ADD_DOT) <= kRedBlocks &&
                         !sh.mfma && h.n_smooth > 0;
   const int n_rz_fused = fused_rz ? wht_partials(g0, sizeof(T), TM_JAC_ADD_DOT) : kRedBlocks;

@@ -76,10 +76,10 @@ const char* et_version(void);
  * ET_TUNE_LEVEL1_MF: 1 (default) = the first coarse level of 3-D hierarchies with even
  *   element counts is applied matrix-free from the element densities; 0 = stored stencil.
  *   Read by et_mg_create.
- * ET_TUNE_FUSED_RZ: 1 (default) = in PCG with FP32 V-cycles on the transformed-basis
- *   operator, the last fine-grid sweep of each V-cycle also sums r.z per workgroup,
- *   r taken as the V-cycle's FP32 copy (no separate pass over r and z); 0 = separate
- *   reduction with r in FP64.  Read by each solve. */
+ * ET_TUNE_FUSED_RZ: 1 (default) = in PCG with the transformed-basis operator, the
+ *   last fine-grid sweep of each V-cycle also sums r.z per workgroup, r taken in the
+ *   V-cycle's precision (FP32 V-cycles: their FP32 copy of r); no separate pass over r
+ *   and z.  0 = separate reduction with r in FP64.  Read by each solve. */
 enum {
   ET_TUNE_TILED_JC = 0,
   ET_TUNE_MFMA_JC = 1,

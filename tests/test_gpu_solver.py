@@ -211,8 +211,8 @@ def test_coarse_solve_matches_dense(coarse_lib, precision, nel):
 @pytest.mark.parametrize("maxiter", [1000, 7])
 @pytest.mark.parametrize("fused_rz", [1, 0])
 def test_pcg_vector_updates(precision, maxiter, fused_rz):
-    """PCG with r.z summed inside the last fine-grid sweep (ET_TUNE_FUSED_RZ = 1, FP32 V-cycles)
-    or by its own pass (0), and with x += alpha p deferred to the p update: the returned x must satisfy
+    """PCG with r.z summed inside the last fine-grid sweep (ET_TUNE_FUSED_RZ = 1) or by its own
+    pass (0), and with x += alpha p deferred to the p update: the returned x must satisfy
     |f - A x| / |f| = the reported residual, whether the loop ends on the tolerance or on
     maxiter, and both settings give the same iterates."""
     ET = _edgeto()
