@@ -257,3 +257,52 @@ def test_pcg_vector_updates(precision, maxiter, fused_rz):
 
 
 _PCG_RUNS = {}
+
+
+# (ET_TUNE_WHT_THREADS_F32, ET_TUNE_WHT_THREADS_F64, ET_TUNE_WHT_SLICES_F32, ET_TUNE_WHT_SLICES_F64)
+_TILES = {"default": (1024, 512, 16, 8), "f32_half": (512, 512, 8, 8), "f64_full": (1024, 512, 16, 16),
+          "f64_256": (1024, 256, 16, 8)}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("tiles", ["default", "f32_half", "f64_full", "f64_256"])
+def test_solve_tile_variants(precision, tiles):
+    """The V-cycle's fused fine-grid modes (two zero-guess sweeps in one pass, the last sweep with
+    the r.z partials) on every transformed-basis tile shape: the solution of a clamped cantilever
+    with random densities against a direct solve, and the iteration count of the default tile."""
+    ET = _edgeto()
+    from edgeto_amd import _lib as L
+    knobs = (10, 11, 15, 16)
+    old = [L.lib().et_tune_get(k) for k in knobs]
+    nel = (24, 12, 12)
+    for k, v in zip(knobs, _TILES[tiles]):
+        L.call("et_tune_set", k, v)
+    try:
+        mesh, kernel, filt, clamp, load = _cantilever_3d(ET, nel)
+        rng = np.random.default_rng(17)
+        E = rng.uniform(0.05, 1.0, mesh.n_el)
+        kernel.set_rho(torch.tensor(E, device="cuda"))
+        tp = ET.CuTopOpt(mesh, ET.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=0.3), kernel,
+                         None, filt, verbose=False)
+        tp.add_BC_nodal(clamp, np.array([[1, 1, 1]]))
+        tp.add_F_nodal(load, np.array([[0.0, -1.0 / len(load), 0.0]]))
+        solver = ET.CuMultiGrid(mesh, kernel, tol=1e-10, n_smooth=2, n_level=2, precision=precision)
+        x, res = solver.solve(tp.f, use_last=False)
+        iters = solver.last_iterations
+    finally:
+        for k, v in zip(knobs, old):
+            L.call("et_tune_set", k, v)
+    assert res < 1e-10
+    prob = _oracle_problem(nel, 0.3, 2.0, clamp, load, -1.0 / len(load))
+    A = oops.assemble(prob.K0, prob.elements, E, prob.cons, prob.n_dof)
+    import scipy.sparse.linalg as spla
+    xd = spla.spsolve(A.tocsc(), prob.f)
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(xg - xd) / np.linalg.norm(xd) < 1e-7
+    runs = _TILE_RUNS.setdefault(precision, {})
+    runs[tiles] = iters
+    if "default" in runs:
+        assert abs(runs["default"] - iters) <= 1
+
+
+_TILE_RUNS = {}
