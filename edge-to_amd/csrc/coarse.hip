@@ -286,7 +286,11 @@ __global__ __launch_bounds__(256) void bchol_panel_kernel(double* A, int ld, int
 // the diagonal tile kb (L_21: the nr panel tiles).  One workgroup per lower tile
 // (i, j), j <= i; each sums over the 64 panel columns in order.  Upper halves of
 // the diagonal tiles are left as they are (never read as L).
-__global__ __launch_bounds__(256) void bchol_syrk_kernel(double* A, int ld, int kb) {
+//
+// The workgroup of tile (0, 0) -- the next diagonal tile -- then factors it
+// (chol_tile64 into Wk), so the next step's diagonal factor overlaps the other
+// workgroups' updates instead of waiting for a launch of its own.
+__global__ __launch_bounds__(256) void bchol_syrk_kernel(double* A, int ld, int kb, double* Wk, int* info) {
   __shared__ double Xs[NB][NB + 1];  // L_i [row][m]
   __shared__ double Ys[NB][NB + 1];  // L_j [row][m]
   int i = 0;
@@ -318,6 +322,10 @@ __global__ __launch_bounds__(256) void bchol_syrk_kernel(double* A, int ld, int 
 #pragma unroll
     for (int p = 0; p < 4; ++p)
       if (i > j || r0 + p >= c0 + q) A[(int64_t)(rj + c0 + q) * ld + ri + r0 + p] -= acc[p][q];
+  if (blockIdx.x == 0) {
+    __syncthreads();  // the updated tile is read back by other threads of this workgroup
+    chol_tile64(A, ld, o + NB, Wk, info);
+  }
 }
 
 // Z = Y row-major (Z[i ld + j] = Y_ij) for the tiles on and below the diagonal
@@ -485,14 +493,15 @@ int CoarseSolver::factor(const Grid& g, int dof, const double* S, hipStream_t s)
     const int64_t bw = g.dim == 3 ? (int64_t)dof * ((int64_t)g.nx() * g.nz() + g.nz() + 1) + dof - 1
                                   : (int64_t)dof * (g.nx() + 1) + dof - 1;
     const int nbd = (int)std::min<int64_t>((bw + NB - 1) / NB, nbt - 1);
+    // tile 0 here; tile kb + 1 by the band update of step kb
+    bchol_diag_kernel<<<1, kCholTileThreads, 0, s>>>(A, ld, 0, Wd, info);
+    ET_CHECK_LAUNCH();
     for (int kb = 0; kb < nbt; ++kb) {
       const int nr = std::min(nbd, nbt - kb - 1);
-      bchol_diag_kernel<<<1, kCholTileThreads, 0, s>>>(A, ld, kb * NB, Wd, info);
-      ET_CHECK_LAUNCH();
       if (nr > 0) {
         bchol_panel_kernel<<<nr, 256, 0, s>>>(A, ld, kb, Wd);
         ET_CHECK_LAUNCH();
-        bchol_syrk_kernel<<<nr * (nr + 1) / 2, 256, 0, s>>>(A, ld, kb);
+        bchol_syrk_kernel<<<nr * (nr + 1) / 2, kCholTileThreads, 0, s>>>(A, ld, kb, Wd, info);
         ET_CHECK_LAUNCH();
       }
     }
