@@ -34,6 +34,23 @@ using et::Grid;
 
 namespace {
 
+// The 3-D fine-grid kernels need an element matrix with the brick's reflection
+// structure (k0_structure); K0 arrives in device memory here, so it is read
+// back first (this synchronises the stream).  2-D grids always use the
+// general kernel.
+template <typename T>
+static int fine_allowed(const Grid& g, const T* K0, hipStream_t stream, bool* ok) {
+  *ok = false;
+  if (g.dim != 3) return ET_OK;
+  T h[576];
+  ET_CHECK_HIP(hipMemcpyAsync(h, K0, sizeof(h), hipMemcpyDeviceToHost, stream));
+  ET_CHECK_HIP(hipStreamSynchronize(stream));
+  double d[576];
+  for (int t = 0; t < 576; ++t) d[t] = (double)h[t];
+  *ok = et::k0_fast3d(et::k0_structure(3, d, sizeof(T) == 8 ? et::kK0Rtol : et::kK0RtolF32));
+  return ET_OK;
+}
+
 template <typename T>
 static int residual_impl(const char* name, int dim, const int* nel, const T* K0, const T* E,
                          const uint8_t* cons, const T* x, const T* b, T* r, hipStream_t stream) {
@@ -41,7 +58,9 @@ static int residual_impl(const char* name, int dim, const int* nel, const T* K0,
   ET_TRY(et::make_grid(dim, nel, &g));
   ET_REQUIRE(K0 && E && x && b && r, name);
   ET_REQUIRE(r != x && r != b, "residual: r must not alias x or b");
-  return et::level0_op<T>(1, g, K0, E, cons, x, b, nullptr, nullptr, 1.0, nullptr, r, stream);
+  bool fine = false;
+  ET_TRY(fine_allowed(g, K0, stream, &fine));
+  return et::level0_op<T>(1, g, K0, E, cons, x, b, nullptr, nullptr, 1.0, nullptr, r, stream, fine);
 }
 
 template <typename T>
@@ -53,7 +72,9 @@ static int jacobi_impl(const char* name, int dim, const int* nel, const T* K0, c
   ET_REQUIRE(K0 && E && x && b && Dinv && y, name);
   ET_REQUIRE(y != x && y != b && y != Dinv, "jacobi: y must not alias x, b or Dinv");
   ET_REQUIRE(std::isfinite(omega), "jacobi: omega must be finite");
-  return et::level0_op<T>(2, g, K0, E, cons, x, b, Dinv, nullptr, omega, nullptr, y, stream);
+  bool fine = false;
+  ET_TRY(fine_allowed(g, K0, stream, &fine));
+  return et::level0_op<T>(2, g, K0, E, cons, x, b, Dinv, nullptr, omega, nullptr, y, stream, fine);
 }
 
 }  // namespace
@@ -130,6 +151,15 @@ int et_tune_get(int param) {
   return -1;
 }
 
+int et_k0_structure(int dim, const double* K0_host, int* flags) {
+  ET_GUARD({
+    ET_REQUIRE(dim == 2 || dim == 3, "et_k0_structure: dim must be 2 or 3");
+    ET_REQUIRE(K0_host && flags, "et_k0_structure: null pointer");
+    *flags = et::k0_structure(dim, K0_host, et::kK0Rtol);
+    return ET_OK;
+  })
+}
+
 int et_generate_elements(int dim, const int* nel, int32_t* elements, hipStream_t stream) {
   ET_GUARD({
     Grid g;
@@ -146,7 +176,9 @@ int et_matvec_f64(int dim, const int* nel, const double* K0, const double* E, co
     ET_TRY(et::make_grid(dim, nel, &g));
     ET_REQUIRE(K0 && E && u && y, "et_matvec_f64: null pointer");
     ET_REQUIRE(u != y, "et_matvec_f64: u and y must not alias");
-    return et::matvec<double>(g, K0, E, cons, u, y, stream);
+    bool fine = false;
+    ET_TRY(fine_allowed(g, K0, stream, &fine));
+    return et::matvec<double>(g, K0, E, cons, u, y, stream, fine);
   })
 }
 
@@ -157,7 +189,9 @@ int et_matvec_f32(int dim, const int* nel, const float* K0, const float* E, cons
     ET_TRY(et::make_grid(dim, nel, &g));
     ET_REQUIRE(K0 && E && u && y, "et_matvec_f32: null pointer");
     ET_REQUIRE(u != y, "et_matvec_f32: u and y must not alias");
-    return et::matvec<float>(g, K0, E, cons, u, y, stream);
+    bool fine = false;
+    ET_TRY(fine_allowed(g, K0, stream, &fine));
+    return et::matvec<float>(g, K0, E, cons, u, y, stream, fine);
   })
 }
 
@@ -193,6 +227,8 @@ int et_matvec_f32_mfma(int dim, const int* nel, const double* K0_host, const flo
     ET_REQUIRE(K0_host && K0 && E && u && y, "et_matvec_f32_mfma: null pointer");
     ET_REQUIRE(u != y, "et_matvec_f32_mfma: u and y must not alias");
     ET_REQUIRE(et::mfma_supported(g), "et_matvec_f32_mfma: 3-D grids only (nz <= ~290)");
+    ET_REQUIRE(et::k0_fast3d(et::k0_structure(3, K0_host, et::kK0Rtol)),
+               "et_matvec_f32_mfma: K0 lacks the brick's reflection structure (use et_matvec_f32)");
     return et::op0_mfma(0, g, (int64_t)g.ey * g.ez, g.ez, et::k0_scale(K0_host, 576), K0, E, cons, u,
                         nullptr, nullptr, nullptr, 1.0, nullptr, y, stream);
   })
@@ -226,6 +262,18 @@ int et_sensitivity_f64(int dim, const int* nel, const double* K0, const double* 
     ET_TRY(et::make_grid(dim, nel, &g));
     ET_REQUIRE(K0 && U && xt && out, "et_sensitivity_f64: null pointer");
     return et::elem_energy(g, K0, U, xt, penal, void_, 1, out, stream);
+  })
+}
+
+int et_fea_integrals_f64(int dim, const int* nel, const double* K0, const double* D, const double* B,
+                         const double* E, const double* U, double* strain, double* stress,
+                         double* energy, double* von_mises, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(K0 && D && B && E && U && strain && stress && energy,
+               "et_fea_integrals_f64: null pointer");
+    return et::fea_integrals(g, K0, D, B, E, U, strain, stress, energy, von_mises, stream);
   })
 }
 
@@ -266,6 +314,18 @@ int et_filter_adj_f64(int dim, const int* nel, int n_off, const int32_t* offsets
     ET_REQUIRE(offsets && w && norm && v && y, "et_filter_adj_f64: null pointer");
     ET_REQUIRE(v != y, "et_filter_adj_f64: v and y must not alias");
     return et::filter_adj(g, n_off, offsets, w, norm, v, y, stream);
+  })
+}
+
+int et_simp_adjoint_f64(int dim, const int* nel, const double* K0, const double* U, const double* xt,
+                        double penal, double void_, int n_off, const int32_t* offsets, const double* w,
+                        const double* norm, double* out, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(K0 && U && xt && offsets && w && norm && out, "et_simp_adjoint_f64: null pointer");
+    ET_REQUIRE(out != xt && out != norm, "et_simp_adjoint_f64: out must not alias xt or norm");
+    return et::simp_adjoint(g, K0, U, xt, penal, void_, n_off, offsets, w, norm, out, stream);
   })
 }
 
@@ -366,6 +426,39 @@ int et_sqdist_f64(int64_t n, const double* a, const double* b, double* out, hipS
     static thread_local double* partial = nullptr;
     if (partial == nullptr) ET_CHECK_HIP(hipMalloc(&partial, et::kRedBlocks * sizeof(double)));
     return et::sqdist(n, a, b, partial, out, stream);
+  })
+}
+
+int64_t et_krylov_work_size(int64_t n_dof, int restart) { return et::krylov_work_size(n_dof, restart); }
+
+int et_cg_f64(int dim, const int* nel, const double* K0, const double* E, const uint8_t* cons, const double* b,
+              double* x, double tol, int maxiter, void* work, int64_t work_bytes, int* iters, double* residual,
+              hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(K0 && E && b && x && work, "et_cg_f64: null pointer");
+    ET_REQUIRE(b != x, "et_cg_f64: b and x must not alias");
+    ET_REQUIRE(std::isfinite(tol) && tol >= 0, "et_cg_f64: tol must be finite and >= 0");
+    bool fine = false;
+    ET_TRY(fine_allowed(g, K0, stream, &fine));
+    return et::cg_solve(g, K0, E, cons, fine, b, x, tol, maxiter, work, work_bytes, iters, residual, stream);
+  })
+}
+
+int et_gmres_f64(int dim, const int* nel, const double* K0, const double* E, const uint8_t* cons, const double* b,
+                 double* x, double tol, int restart, int maxiter, void* work, int64_t work_bytes, int* iters,
+                 double* residual, hipStream_t stream) {
+  ET_GUARD({
+    Grid g;
+    ET_TRY(et::make_grid(dim, nel, &g));
+    ET_REQUIRE(K0 && E && b && x && work, "et_gmres_f64: null pointer");
+    ET_REQUIRE(b != x, "et_gmres_f64: b and x must not alias");
+    ET_REQUIRE(std::isfinite(tol) && tol >= 0, "et_gmres_f64: tol must be finite and >= 0");
+    bool fine = false;
+    ET_TRY(fine_allowed(g, K0, stream, &fine));
+    return et::gmres_solve(g, K0, E, cons, fine, b, x, tol, restart, maxiter, work, work_bytes, iters, residual,
+                           stream);
   })
 }
 

@@ -14,10 +14,13 @@ namespace et {
 void set_error(const std::string& msg);
 const char* last_error();
 
+// On failure the runtime's last-error slot is cleared as well, so a failed
+// allocation is not reported again by the next, unrelated launch check.
 #define ET_CHECK_HIP(expr)                                                           \
   do {                                                                               \
     hipError_t _e = (expr);                                                          \
     if (_e != hipSuccess) {                                                          \
+      (void)hipGetLastError();                                                       \
       ::et::set_error(std::string(#expr " failed: ") + hipGetErrorString(_e));       \
       return ET_EHIP;                                                                \
     }                                                                                \

@@ -350,11 +350,11 @@ __global__ __launch_bounds__(256) void l1_fix_kernel(const Grid g1, const int* _
 
 // ------------------------------------------------------------ host ----
 
-int Level1MF::init(const Grid& fine, const double* K0_host, bool mixed_, hipStream_t s) {
-  g0 = fine;
-  g1 = fine.coarse();
-  mixed = mixed_;
-  // Mh = H M_0 H / 64, M_0 = W_0^T K0 W_0 in corner-bit order
+namespace {
+inline int l1_at(int B, int c, int D, int d) { return ((B * 3 + c) * 8 + D) * 3 + d; }
+
+// Mh = H M_0 H / 64, M_0 = W_0^T K0 W_0 in corner-bit order, [M][c][N][d]
+std::vector<double> level1_mh(const double* K0_host) {
   double Kb[8][3][8][3];
   for (int b = 0; b < 8; ++b)
     for (int b2 = 0; b2 < 8; ++b2)
@@ -371,7 +371,6 @@ int Level1MF::init(const Grid& fine, const double* K0_host, bool mixed_, hipStre
       W[b][B] = w;
     }
   std::vector<double> M0(8 * 3 * 8 * 3, 0.0), Mh(8 * 3 * 8 * 3, 0.0);
-  auto at = [](int B, int c, int D, int d) { return ((B * 3 + c) * 8 + D) * 3 + d; };
   for (int B = 0; B < 8; ++B)
     for (int c = 0; c < 3; ++c)
       for (int D = 0; D < 8; ++D)
@@ -379,7 +378,7 @@ int Level1MF::init(const Grid& fine, const double* K0_host, bool mixed_, hipStre
           double v = 0.0;
           for (int b = 0; b < 8; ++b)
             for (int b2 = 0; b2 < 8; ++b2) v += W[b][B] * Kb[b][c][b2][d] * W[b2][D];
-          M0[at(B, c, D, d)] = v;
+          M0[l1_at(B, c, D, d)] = v;
         }
   auto h = [](int m, int b) { return (__builtin_popcount(m & b) & 1) ? -1.0 : 1.0; };
   for (int M = 0; M < 8; ++M)
@@ -388,9 +387,30 @@ int Level1MF::init(const Grid& fine, const double* K0_host, bool mixed_, hipStre
         for (int d = 0; d < 3; ++d) {
           double v = 0.0;
           for (int B = 0; B < 8; ++B)
-            for (int D = 0; D < 8; ++D) v += h(M, B) * M0[at(B, c, D, d)] * h(N, D);
-          Mh[at(M, c, N, d)] = v / 64.0;
+            for (int D = 0; D < 8; ++D) v += h(M, B) * M0[l1_at(B, c, D, d)] * h(N, D);
+          Mh[l1_at(M, c, N, d)] = v / 64.0;
         }
+  return Mh;
+}
+}  // namespace
+
+bool level1_terms_cover(const double* K0_host, double rtol) {
+  const std::vector<double> Mh = level1_mh(K0_host);
+  std::vector<char> kept(Mh.size(), 0);
+  for (int t = 0; t < kL1Terms; ++t) kept[l1_at(kL1Term[t][1], kL1Term[t][0], kL1Term[t][3], kL1Term[t][2])] = 1;
+  double mx = 0.0;
+  for (double v : Mh) mx = std::fmax(mx, std::fabs(v));
+  for (size_t i = 0; i < Mh.size(); ++i)
+    if (!kept[i] && std::fabs(Mh[i]) > rtol * mx) return false;
+  return true;
+}
+
+int Level1MF::init(const Grid& fine, const double* K0_host, bool mixed_, hipStream_t s) {
+  g0 = fine;
+  g1 = fine.coarse();
+  mixed = mixed_;
+  const std::vector<double> Mh = level1_mh(K0_host);
+  auto at = l1_at;
   std::vector<double> cd(kL1Terms);
   std::vector<float> cfl(kL1Terms);
   for (int t = 0; t < kL1Terms; ++t) {

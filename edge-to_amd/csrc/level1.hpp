@@ -35,6 +35,9 @@ struct Level1MF {
 };
 
 bool level1_supported(const Grid& fine, int n_level);
+// whether the fixed term list of the element product holds every entry of
+// H M_0 H / 64 above rtol * max for this element matrix (host)
+bool level1_terms_cover(const double* K0_host, double rtol);
 void set_level1_mf(int on);
 int get_level1_mf();
 

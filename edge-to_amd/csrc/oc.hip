@@ -1,22 +1,65 @@
-// Optimality-criteria design update on the device.
+// Optimality-criteria design update on the device, with no host round trip.
 //
-// The reference bisects lambda in [1e-9, 1e9] with a host round trip per
-// step.  Here each step is two kernels (a partial volume sum at the current
-// midpoint, then a one-block kernel that finalises the sum, moves l1/l2 and
-// raises a `done` flag); steps are enqueued in chunks and every kernel is a
-// no-op once `done` is set, so the host synchronises once per chunk instead
-// of once per step.  The midpoint sequence and the decisions are those of
-// the reference; the new design is evaluated once, at the last midpoint.
+// The reference bisects lambda in [1e-9, 1e9] (TopOpt/_cuda.py:242-280) with a
+// host synchronisation per step.  Here one pass evaluates the volume at every
+// midpoint of the next kOcDepth bisection levels at once (the 2^kOcDepth - 1
+// nodes of the subtree below the current bracket, formed exactly as the
+// sequential 0.5 * (l1 + l2) would form them), and a one-block kernel then
+// walks the subtree with the reference's decisions and stopping rule.  A fixed
+// number of passes is enqueued, enough for the longest bisection the bracket
+// and the 1e-4 stopping rule allow (73 steps); passes after the stop return at
+// once.  The new design is evaluated once, at the last midpoint tried.  Every
+// kernel is stream-ordered: the whole update can be captured in a graph.
+#include <algorithm>
+
 #include "common.hpp"
 #include "blas.hpp"
 
 namespace et {
 
-struct OcScalars {
+// Bisection state at the start of the work buffer (callers may read `steps`
+// there once the stream has reached the update).
+struct OcState {
   double l1, l2, lmid, sabs;
   int done, steps;
 };
+static_assert(sizeof(OcState) == 40, "OcState layout is part of the C ABI");
 
+constexpr int kOcDepth = 3;                       // bisection levels per pass
+constexpr int kOcCand = (1 << kOcDepth) - 1;      // midpoints evaluated per pass
+constexpr int kOcPasses = 26;                     // 78 >= 73 steps: every bisection ends
+constexpr int64_t kOcStateBytes = 256;            // state slot (keeps ocp 256-byte aligned)
+
+__host__ __device__ inline bool oc_continue(double l1, double l2) { return (l2 - l1) / (l2 + l1) > 1e-4; }
+
+// midpoints of the subtree below [l1, l2] in heap order (node i: left child
+// 2i+1 = [lo, m_i], right child 2i+2 = [m_i, hi])
+__device__ inline void oc_midpoints(double l1, double l2, double (&m)[kOcCand]) {
+  double lo[kOcCand], hi[kOcCand];
+  lo[0] = l1;
+  hi[0] = l2;
+#pragma unroll
+  for (int i = 0; i < kOcCand; ++i) {
+    m[i] = 0.5 * (lo[i] + hi[i]);
+    if (2 * i + 2 < kOcCand) {
+      lo[2 * i + 1] = lo[i];
+      hi[2 * i + 1] = m[i];
+      lo[2 * i + 2] = m[i];
+      hi[2 * i + 2] = hi[i];
+    }
+  }
+}
+
+__device__ __forceinline__ double oc_value(double x, double ocp_raw, bool all_zero, double move,
+                                           double lmid) {
+  const double xu = fmin(fmax(x + move, 0.0), 1.0);
+  const double xl = fmin(fmax(x - move, 0.0), 1.0);
+  const double p = all_zero ? 1e-3 : fmax(1e-12, ocp_raw);
+  return fmax(0.0, fmax(xl, fmin(1.0, fmin(xu, p / lmid))));
+}
+
+// ocp = x * sqrt(max(-dc/A, 0)) with NaN -> 0 (SingleMaterial.__ocP__,
+// MaterialModels.py:83-90), and per-block sums of |ocp|
 __global__ __launch_bounds__(256) void oc_prep_kernel(int64_t n, const double* __restrict__ x,
                                                       const double* __restrict__ dc, double A,
                                                       double* __restrict__ ocp,
@@ -35,117 +78,126 @@ __global__ __launch_bounds__(256) void oc_prep_kernel(int64_t n, const double* _
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ void oc_init_kernel(const double* __restrict__ partial, OcScalars* sc) {
+__global__ void oc_init_kernel(int nb, const double* __restrict__ partial, OcState* st) {
   double s = 0.0;
-  for (int t = threadIdx.x; t < kRedBlocks; t += blockDim.x) s += partial[t];
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) s += partial[t];
   s = block_sum256(s);
   if (threadIdx.x == 0) {
-    sc->sabs = s;
-    sc->l1 = 1e-9;
-    sc->l2 = 1e9;
-    sc->lmid = 0.5 * (sc->l1 + sc->l2);
-    sc->done = !((sc->l2 - sc->l1) / (sc->l2 + sc->l1) > 1e-4);
-    sc->steps = 0;
+    st->sabs = s;
+    st->l1 = 1e-9;
+    st->l2 = 1e9;
+    st->lmid = 0.5 * (st->l1 + st->l2);
+    st->done = !oc_continue(st->l1, st->l2);
+    st->steps = 0;
   }
 }
 
-__device__ __forceinline__ double oc_value(double x, double ocp_raw, bool all_zero, double move,
-                                           double lmid) {
-  const double xu = fmin(fmax(x + move, 0.0), 1.0);
-  const double xl = fmin(fmax(x - move, 0.0), 1.0);
-  const double p = all_zero ? 1e-3 : fmax(1e-12, ocp_raw);
-  return fmax(0.0, fmax(xl, fmin(1.0, fmin(xu, p / lmid))));
-}
-
-__global__ __launch_bounds__(256) void oc_step_kernel(int64_t n, const double* __restrict__ x,
+// per-block volume sums sum(x_new(m_c) * A) at every midpoint m_c of the pass
+__global__ __launch_bounds__(256) void oc_pass_kernel(int64_t n, const double* __restrict__ x,
                                                       const double* __restrict__ ocp, double A,
-                                                      double move, const OcScalars* sc,
-                                                      double* __restrict__ partial) {
-  if (sc->done) return;
-  const double lmid = 0.5 * (sc->l1 + sc->l2);
-  const bool all_zero = sc->sabs == 0.0;
-  double s = 0.0;
+                                                      double move, const OcState* __restrict__ st,
+                                                      double* __restrict__ partial, int nb) {
+  if (st->done) return;
+  double m[kOcCand];
+  oc_midpoints(st->l1, st->l2, m);
+  const bool all_zero = st->sabs == 0.0;
+  double s[kOcCand];
+#pragma unroll
+  for (int c = 0; c < kOcCand; ++c) s[c] = 0.0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * blockDim.x)
-    s += oc_value(x[t], ocp[t], all_zero, move, lmid) * A;
-  s = block_sum256(s);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const double xv = x[t], pv = ocp[t];
+#pragma unroll
+    for (int c = 0; c < kOcCand; ++c) s[c] += oc_value(xv, pv, all_zero, move, m[c]) * A;
+  }
+#pragma unroll
+  for (int c = 0; c < kOcCand; ++c) {
+    const double v = block_sum256(s[c]);
+    if (threadIdx.x == 0) partial[c * nb + blockIdx.x] = v;
+  }
 }
 
-__global__ void oc_decide_kernel(const double* __restrict__ partial, double vol, double vf,
-                                 OcScalars* sc) {
-  if (sc->done) return;
-  double s = 0.0;
-  for (int t = threadIdx.x; t < kRedBlocks; t += blockDim.x) s += partial[t];
-  s = block_sum256(s);
+// sums each midpoint's volume (fixed order) and takes up to kOcDepth of the
+// reference's steps: l2 = lmid if the volume fraction is within vf, else l1 = lmid
+__global__ void oc_decide_kernel(const double* __restrict__ partial, int nb, double vol, double vf,
+                                 OcState* st) {
+  if (st->done) return;
+  __shared__ double tot[kOcCand];
+  for (int c = 0; c < kOcCand; ++c) {
+    double s = 0.0;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) s += partial[c * nb + t];
+    s = block_sum256(s);
+    if (threadIdx.x == 0) tot[c] = s;
+  }
   if (threadIdx.x == 0) {
-    const double lmid = 0.5 * (sc->l1 + sc->l2);
-    sc->lmid = lmid;
-    if (s / vol <= vf)
-      sc->l2 = lmid;
-    else
-      sc->l1 = lmid;
-    sc->steps += 1;
-    sc->done = !((sc->l2 - sc->l1) / (sc->l2 + sc->l1) > 1e-4);
+    double m[kOcCand];
+    double l1 = st->l1, l2 = st->l2, lmid = st->lmid;
+    oc_midpoints(l1, l2, m);
+    int node = 0, steps = st->steps, done = 0;
+    for (int d = 0; d < kOcDepth; ++d) {
+      lmid = m[node];
+      const bool feasible = tot[node] / vol <= vf;
+      if (feasible)
+        l2 = lmid;
+      else
+        l1 = lmid;
+      ++steps;
+      if (!oc_continue(l1, l2)) {
+        done = 1;
+        break;
+      }
+      node = feasible ? 2 * node + 1 : 2 * node + 2;
+    }
+    st->l1 = l1;
+    st->l2 = l2;
+    st->lmid = lmid;
+    st->steps = steps;
+    st->done = done;
   }
 }
 
 __global__ __launch_bounds__(256) void oc_final_kernel(int64_t n, const double* __restrict__ x,
                                                        const double* __restrict__ ocp, double move,
-                                                       const OcScalars* sc,
+                                                       const OcState* __restrict__ st,
                                                        double* __restrict__ xn) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  xn[t] = oc_value(x[t], ocp[t], sc->sabs == 0.0, move, sc->lmid);
+  xn[t] = oc_value(x[t], ocp[t], st->sabs == 0.0, move, st->lmid);
 }
 
 int64_t oc_work_size(int64_t n) {
-  return n * (int64_t)sizeof(double) + kRedBlocks * (int64_t)sizeof(double) + 256;
+  return kOcStateBytes + n * (int64_t)sizeof(double) + kOcCand * kRedBlocks * (int64_t)sizeof(double);
 }
 
 int oc_update(int64_t n, const double* x, const double* dc, double A, double vol, double vf,
               double move, double* xn, void* work, int* steps, hipStream_t s) {
   ET_REQUIRE(n > 0 && x && dc && xn && work, "oc_update: bad arguments");
   ET_REQUIRE(A > 0 && vol > 0, "oc_update: element measure and volume must be positive");
-  double* ocp = static_cast<double*>(work);
+  ET_REQUIRE(xn != x && xn != dc, "oc_update: x_new must not alias x or dc");
+  ET_REQUIRE((reinterpret_cast<uintptr_t>(work) & 7) == 0, "oc_update: work must be 8-byte aligned");
+  OcState* st = static_cast<OcState*>(work);
+  double* ocp = reinterpret_cast<double*>(static_cast<char*>(work) + kOcStateBytes);
   double* partial = ocp + n;
-  OcScalars* sc = reinterpret_cast<OcScalars*>(partial + kRedBlocks);
-  // pinned read-back slot, allocated once per host thread and kept for the process
-  static thread_local OcScalars* h = nullptr;
-  if (h == nullptr) ET_CHECK_HIP(hipHostMalloc(&h, sizeof(OcScalars)));
-  int status = ET_OK;
-  do {
-    oc_prep_kernel<<<kRedBlocks, 256, 0, s>>>(n, x, dc, A, ocp, partial);
-    oc_init_kernel<<<1, 256, 0, s>>>(partial, sc);
-    constexpr int kChunk = 12;
-    for (int round = 0; round < 40; ++round) {  // bisection needs < 120 steps
-      for (int c = 0; c < kChunk; ++c) {
-        oc_step_kernel<<<kRedBlocks, 256, 0, s>>>(n, x, ocp, A, move, sc, partial);
-        oc_decide_kernel<<<1, 256, 0, s>>>(partial, vol, vf, sc);
-      }
-      if (hipMemcpyAsync(h, sc, sizeof(OcScalars), hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess) {
-        set_error("oc_update: HIP error during bisection");
-        status = ET_EHIP;
-        break;
-      }
-      if (h->done) break;
-    }
-    if (status != ET_OK) break;
-    if (!h->done) {
+  const int nb = (int)std::min<int64_t>(kRedBlocks, (n + 255) / 256);
+  oc_prep_kernel<<<nb, 256, 0, s>>>(n, x, dc, A, ocp, partial);
+  oc_init_kernel<<<1, 256, 0, s>>>(nb, partial, st);
+  for (int pass = 0; pass < kOcPasses; ++pass) {
+    oc_pass_kernel<<<nb, 256, 0, s>>>(n, x, ocp, A, move, st, partial, nb);
+    oc_decide_kernel<<<1, 256, 0, s>>>(partial, nb, vol, vf, st);
+  }
+  oc_final_kernel<<<grid_for(n), kBlock, 0, s>>>(n, x, ocp, move, st, xn);
+  ET_CHECK_LAUNCH();
+  if (steps != nullptr) {
+    OcState h{};
+    ET_CHECK_HIP(hipMemcpyAsync(&h, st, sizeof(OcState), hipMemcpyDeviceToHost, s));
+    ET_CHECK_HIP(hipStreamSynchronize(s));
+    *steps = h.steps;
+    if (!h.done) {
       set_error("oc_update: bisection did not terminate");
-      status = ET_ENOCONV;
-      break;
+      return ET_ENOCONV;
     }
-    oc_final_kernel<<<grid_for(n), kBlock, 0, s>>>(n, x, ocp, move, sc, xn);
-    if (hipGetLastError() != hipSuccess) {
-      set_error("oc_update: launch failed");
-      status = ET_EHIP;
-      break;
-    }
-    if (steps) *steps = h->steps;
-  } while (0);
-  return status;
+  }
+  return ET_OK;
 }
 
 // sum (a - b)^2, used for the RMS design change of the SIMP loop

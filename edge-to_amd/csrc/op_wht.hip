@@ -536,9 +536,23 @@ static int g_wht_wgs = 1;
 // 16-slice tiles (fastest in both precisions)
 static int g_wht_nt[2] = {1024, 512};
 static int g_wht_ns[2] = {16, 8};
+// per-host-thread overrides (0 = the process-wide value): lets concurrent
+// solves on different threads use different tiles
+static thread_local int t_wht_nt[2] = {0, 0};
+static thread_local int t_wht_ns[2] = {0, 0};
+
+static int wht_nt_eff(int p) { return t_wht_nt[p] ? t_wht_nt[p] : g_wht_nt[p]; }
+static int wht_ns_eff(int p) { return t_wht_ns[p] ? t_wht_ns[p] : g_wht_ns[p]; }
+
+void set_thread_wht_tiles(const int* nt, const int* ns) {
+  for (int p = 0; p < 2; ++p) {
+    t_wht_nt[p] = nt[p] == 0 ? 0 : ((nt[p] == 1024 || nt[p] == 256) ? nt[p] : 512);
+    t_wht_ns[p] = ns[p] == 0 ? 0 : (ns[p] == 8 ? 8 : 16);
+  }
+}
 
 static int wht_ns(size_t elem_bytes, int mode) {
-  return (mode == TM_Y || mode == TM_YDOT) ? 16 : g_wht_ns[elem_bytes == 8 ? 1 : 0];
+  return (mode == TM_Y || mode == TM_YDOT) ? 16 : wht_ns_eff(elem_bytes == 8 ? 1 : 0);
 }
 
 void set_wht_threads(int is_double, int nt) {
@@ -560,7 +574,7 @@ bool wht_supported(const Grid& g, size_t elem_bytes) {
   if (g.dim != 3 || g.ez > 128 || 3 * g.n_nodes() >= ((int64_t)1 << 31) ||
       3 * g.n_nodes() * (int64_t)elem_bytes >= ((int64_t)1 << 32) || g.n_elem() * (int64_t)elem_bytes >= ((int64_t)1 << 32))
     return false;
-  for (const int ns : {16, g_wht_ns[elem_bytes == 8 ? 1 : 0]}) {
+  for (const int ns : {16, wht_ns_eff(elem_bytes == 8 ? 1 : 0)}) {
     const int tc = wht_tc(g, ns);
     if (!((tc + 1) * g.nz() <= 96 * ns && tc * g.ez <= 64 * ns && (tc - 1) * g.nz() <= 64 * ns &&
           wht_lds_values(g, ns) * elem_bytes <= kWMaxLds))
@@ -602,12 +616,12 @@ static void launch_wht(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, con
                        const T* dotv = nullptr) {
   const int p = sizeof(T) == 8 ? 1 : 0;
   if (wht_ns(sizeof(T), MODE) == 16) {
-    if (g_wht_nt[p] == 1024)
+    if (wht_nt_eff(p) == 1024)
       launch_wht_cfg<T, MODE, 1024, 16>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
     else
       launch_wht_cfg<T, MODE, 512, 16>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
   } else {
-    if (g_wht_nt[p] == 256)
+    if (wht_nt_eff(p) == 256)
       launch_wht_cfg<T, MODE, 256, 8>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
     else
       launch_wht_cfg<T, MODE, 512, 8>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);

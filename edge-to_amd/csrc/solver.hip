@@ -607,8 +607,8 @@ template struct MGHierarchy<float>;
 template <typename T>
 int level0_op(int mode, const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T* x,
               const T* b, const T* Dinv, const double* omega, double wscale, const T* xadd, T* y,
-              hipStream_t s) {
-  if (g.dim == 3 && fine_supported(g, sizeof(T)))
+              hipStream_t s, bool allow_fine) {
+  if (g.dim == 3 && allow_fine && fine_supported(g, sizeof(T)))
     return op0_fine<T>(mode, g, (int64_t)g.ey * g.ez, g.ez, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, s);
   const unsigned grid = grid_for(g.n_nodes());
 #define ET_L0(DIM, MODE) \
@@ -634,10 +634,10 @@ int level0_op(int mode, const Grid& g, const T* K0, const T* E, const uint8_t* c
 
 template int level0_op<double>(int, const Grid&, const double*, const double*, const uint8_t*,
                                const double*, const double*, const double*, const double*, double,
-                               const double*, double*, hipStream_t);
+                               const double*, double*, hipStream_t, bool);
 template int level0_op<float>(int, const Grid&, const float*, const float*, const uint8_t*,
                               const float*, const float*, const float*, const double*, double,
-                              const float*, float*, hipStream_t);
+                              const float*, float*, hipStream_t, bool);
 
 // ------------------------------------------------------------ et_mg ----
 
@@ -700,8 +700,12 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
   }
   n_dof = g0.n_nodes() * dof;
   n_elem = g0.n_elem();
-  sh.tiled = g0.dim == 3 && fine_supported(g0, sizeof(double)) && fine_supported(g0, sizeof(float));
-  sh.mfma = p.precision == 1 && get_mixed_mfma() && mfma_supported(g0);
+  // the fast 3-D fine-grid kernels build K0's blocks from the brick's reflections:
+  // any other element matrix takes the general node-gather kernel
+  k0_flags = k0_structure(p.dim, K0_host, kK0Rtol);
+  sh.tiled = g0.dim == 3 && k0_fast3d(k0_flags) && fine_supported(g0, sizeof(double)) &&
+             fine_supported(g0, sizeof(float));
+  sh.mfma = p.precision == 1 && sh.tiled && get_mixed_mfma() && mfma_supported(g0);
   sh.kscale = k0_scale(K0_host, (p.dim == 3 ? 24 * 24 : 8 * 8));
   const int nd = (p.dim == 3 ? 8 : 4) * dof;
   ET_CHECK_HIP(hipMalloc(&sh.K0d, nd * nd * sizeof(double)));
@@ -739,7 +743,7 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
       g = g.coarse();
     }
   }
-  use_l1 = elem_galerkin && get_level1_mf() && level1_supported(g0, p.n_level);
+  use_l1 = elem_galerkin && get_level1_mf() && level1_supported(g0, p.n_level) && (k0_flags & K0S_LEVEL1);
   if (use_l1) ET_TRY(l1.init(g0, K0_host, p.precision == 1, s));
   sh.l1 = use_l1 ? &l1 : nullptr;
   if (p.precision == 0)
@@ -1000,7 +1004,11 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
     prof.end(s);
     ET_CHECK_LAUNCH();
   }
-  if (!stopped) (void)read_ring(last, &R, &status);
+  if (!stopped) stopped = read_ring(last, &R, &status);
+  if (!stopped && status == ET_OK) {
+    set_error("PCG: maxiter reached without meeting the tolerance");
+    status = ET_ENOCONV;
+  }
   if (status == ET_ENONFINITE) set_error("PCG: non-finite value (p.q or r.r)");
   if (status == ET_EBREAKDOWN) set_error("PCG breakdown: p^T A p <= 0");
   if (iters) *iters = last;

@@ -128,6 +128,7 @@ struct MGSolver {
   uint8_t* eflag = nullptr;    // per-element "touches a constrained DOF"
   Level1MF l1;                 // matrix-free level 1 (3-D, even counts, n_level >= 2)
   bool use_l1 = false;
+  int k0_flags = 0;            // k0_structure() of the element matrix (fast 3-D kernels only if it allows)
 
   double *r = nullptr, *z = nullptr, *p_ = nullptr, *q = nullptr, *diag64 = nullptr;
   double* partial = nullptr;  // 2 x kRedBlocks first-stage reduction sums

@@ -8,9 +8,25 @@ namespace et {
 
 int generate_elements(const Grid& g, int32_t* el, hipStream_t s);
 
+// Structure of a row-major element matrix (host, FP64; k0_structure.cpp):
+// K0S_SYM symmetric; K0S_REFLECT (3-D) blocks invariant under the brick's axis
+// reflections; K0S_WHT (3-D) only the couplings the transformed-basis operator
+// keeps; K0S_LEVEL1 (3-D) the matrix-free first coarse level's terms cover it.
+// Entries count as zero / equal within rtol * max|K0|.
+enum { K0S_SYM = 1, K0S_REFLECT = 2, K0S_WHT = 4, K0S_LEVEL1 = 8 };
+int k0_structure(int dim, const double* K0, double rtol);
+constexpr double kK0Rtol = 1e-12;  // FP64 element matrices
+constexpr double kK0RtolF32 = 1e-6; // element matrices given in FP32
+// the 3-D fine-grid operators (transformed-basis, LDS-tiled, FP16 matrix-core) apply
+inline bool k0_fast3d(int flags) {
+  return (flags & (K0S_SYM | K0S_REFLECT | K0S_WHT)) == (K0S_SYM | K0S_REFLECT | K0S_WHT);
+}
+
+// allow_fine = false keeps 3-D grids on the general node-gather kernel (element
+// matrices without the reflection structure, k0_fast3d)
 template <typename T>
 int matvec(const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T* u, T* y,
-           hipStream_t s);
+           hipStream_t s, bool allow_fine = true);
 template <typename T>
 int diagonal(const Grid& g, const T* K0, const T* E, const uint8_t* cons, T* diag, hipStream_t s);
 
@@ -58,6 +74,8 @@ int get_wht_wgs();
 void set_wht_threads(int is_double, int nt);
 int get_wht_threads(int is_double);
 void set_wht_slices(int is_double, int ns);
+// tiles of the calling host thread's launches ([float, double]; 0 = the process-wide setting)
+void set_thread_wht_tiles(const int* nt, const int* ns);
 int get_wht_slices(int is_double);
 // Fine-grid dispatch: the transformed-basis operator where enabled and
 // supported, else the matrix-core one where enabled, else the LDS-tiled one.
@@ -73,7 +91,7 @@ bool fine_supported(const Grid& g, size_t elem_bytes);
 template <typename T>
 int level0_op(int mode, const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T* x,
               const T* b, const T* Dinv, const double* omega, double wscale, const T* xadd, T* y,
-              hipStream_t s);
+              hipStream_t s, bool allow_fine = true);
 // Matrix-core FP32 variant (FP16 hi/lo MFMA); kscale: power of two >= max|K0|.
 int op0_mfma(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, float kscale, const float* K0,
              const float* E, const uint8_t* cons, const float* x, const float* b, const float* Dinv,
@@ -90,6 +108,11 @@ int get_tiled_jc();
 // mode 0: -u^T K0 u; mode 1: SIMP sensitivity p*clip(xt)^(p-1) * (-u^T K0 u)
 int elem_energy(const Grid& g, const double* K0, const double* U, const double* xt, double penal,
                 double vmin, int mode, double* out, hipStream_t s);
+// strain (n_elem x NB), stress (n_elem x NB), energy (n_elem), von Mises (n_elem, may be
+// null) of every element; NB = 3 (2-D) or 6 (3-D)
+int fea_integrals(const Grid& g, const double* K0, const double* D, const double* B, const double* E,
+                  const double* U, double* strain, double* stress, double* energy, double* vm,
+                  hipStream_t s);
 int simp_forward(int64_t n, const double* xt, double penal, double vmin, int plain, double* E,
                  hipStream_t s);
 int stencil_from_operator(const Grid& g, const double* K0, const double* E, const uint8_t* cons,
@@ -103,6 +126,10 @@ int filter_fwd(const Grid& g, int n_off, const int32_t* off, const double* w, co
 int filter_adj(const Grid& g, int n_off, const int32_t* off, const double* w, const double* norm,
                const double* v, double* y, hipStream_t s);
 
+// y = H^T [p clip(xt, void, 1)^(p-1) (-u_e^T K0 u_e)] in one LDS-halo stencil pass
+int simp_adjoint(const Grid& g, const double* K0, const double* U, const double* xt, double penal, double vmin,
+                 int n_off, const int32_t* off, const double* w, const double* norm, double* y, hipStream_t s);
+
 // transfers (g = fine grid)
 template <typename T>
 // with x0: also x0 = omega * wscale * Dinv * vc (first Jacobi sweep from zero on the coarse grid)
@@ -115,6 +142,15 @@ int prolong_op(const Grid& g, int dof, const T* vc, T* vf, hipStream_t s);
 int galerkin(const Grid& g, int dof, const double* Sf, double* Sc, hipStream_t s);
 template <typename T>
 int stencil_apply(const Grid& g, int dof, const T* S, const T* x, T* y, hipStream_t s);
+
+// unpreconditioned Krylov solvers on the structured operator (krylov.hip)
+int64_t krylov_work_size(int64_t n, int restart);
+int cg_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, bool fine, const double* b,
+             double* x, double tol, int maxiter, void* work, int64_t work_bytes, int* iters, double* residual,
+             hipStream_t s);
+int gmres_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, bool fine,
+                const double* b, double* x, double tol, int restart, int maxiter, void* work, int64_t work_bytes,
+                int* iters, double* residual, hipStream_t s);
 
 // optimality criteria
 int64_t oc_work_size(int64_t n);

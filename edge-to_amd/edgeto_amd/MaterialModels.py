@@ -49,8 +49,8 @@ class SingleMaterial(MaterialModel):
     def __forward__(self, rho, iteration, plain=False, out=None, **_):
         rho = rho.contiguous()
         out = torch.empty_like(rho) if out is None else out
-        _lib.call("et_simp_forward_f64", rho.numel(), _lib.ptr(rho), float(self._pen(iteration)),
-                  float(self.void), 1 if plain else 0, _lib.ptr(out), _lib.stream())
+        _lib.call("et_simp_forward_f64", rho.numel(), _lib.f64(rho), float(self._pen(iteration)),
+                  float(self.void), 1 if plain else 0, _lib.f64(out, rho.numel()), _lib.stream())
         return out
 
     def __backward__(self, rho, iteration, **_):
@@ -59,8 +59,8 @@ class SingleMaterial(MaterialModel):
         clipped = self.__forward__(rho, iteration, plain=True)
         E = torch.empty_like(clipped)
         # p * x^(p-1) = p * (x^p) / x with x >= void > 0
-        _lib.call("et_simp_forward_f64", clipped.numel(), _lib.ptr(clipped), pen - 1.0, 0.0, 0,
-                  _lib.ptr(E), _lib.stream())
+        _lib.call("et_simp_forward_f64", clipped.numel(), _lib.f64(clipped), pen - 1.0, 0.0, 0,
+                  _lib.f64(E, clipped.numel()), _lib.stream())
         return E * pen
 
     def __initvars__(self, nel, dtype=torch.float64, **_):

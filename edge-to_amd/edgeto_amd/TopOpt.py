@@ -107,25 +107,60 @@ class TopOpt:
             self.f[torch.as_tensor(ids * dof + i, device=self.f.device)] = torch.as_tensor(
                 vals, dtype=torch.float64, device=self.f.device)
 
+    # ------------------------------------------------------- post-processing
+    def FEA_integrals(self, rho):
+        """Strain, stress, strain energy and von Mises stress of every element for the design
+        rho (TopOpt/_cuda.py:78-129): E = clip(rho, void, 1) (the material model's plain map),
+        a warm-started solve, then one fused element pass (et_fea_integrals_f64).  Returns
+        (strain (n_el, NB), stress (n_el, NB), strain_energy (n_el,), von_mises (n_el,))."""
+        rho = torch.as_tensor(rho, dtype=torch.float64, device=self.f.device).reshape(-1).contiguous()
+        rho_ = self.material_model(rho, 1e6, plain=True)
+        self.kernel.set_rho(rho_)
+        U, residual = self.solver.solve(self.f, use_last=True)
+        if residual > 1e-5:
+            logger.warning(f"Solver residual is above 1e-5 ({residual:.4e}). Consider higher iterations "
+                           "(rerun this function and more iteration from prior solve will be applied).")
+        mesh = self.mesh
+        ne = rho_.numel()
+        nb = mesh.B_single.shape[0]
+        dev = self.f.device
+        strain = torch.empty((ne, nb), dtype=torch.float64, device=dev)
+        stress = torch.empty((ne, nb), dtype=torch.float64, device=dev)
+        energy = torch.empty(ne, dtype=torch.float64, device=dev)
+        vm = torch.empty(ne, dtype=torch.float64, device=dev)
+        B = mesh.B_single.contiguous()
+        D = mesh.D_single.contiguous()
+        if tuple(D.shape) != (nb, nb) or B.shape[1] != mesh.K_single.shape[0]:
+            raise ValueError("D_single / B_single do not match the element matrix")
+        _lib.call("et_fea_integrals_f64", len(mesh.nel), _lib.nel_arg(mesh.nel), _lib.f64(mesh.K_single),
+                  _lib.f64(D), _lib.f64(B), _lib.f64(rho_, ne), _lib.f64(U, self.kernel.shape[0]),
+                  _lib.f64(strain), _lib.f64(stress), _lib.f64(energy), _lib.f64(vm), _lib.stream())
+        return strain, stress, energy, vm
+
     # --------------------------------------------------------------- OC / loop
     def update_design(self, rho, df):
         rho = rho.reshape(-1).contiguous()
         df = df.reshape(-1).contiguous()
         out = torch.empty_like(rho)
-        steps = ctypes_int()
-        _lib.call("et_oc_update_f64", rho.numel(), _lib.ptr(rho), _lib.ptr(df), float(self.mesh.A_host),
+        n = rho.numel()
+        # asynchronous: the step count stays in the work buffer (int32 at byte 36)
+        _lib.call("et_oc_update_f64", n, _lib.f64(rho), _lib.f64(df, n), float(self.mesh.A_host),
                   float(self.mesh.volume), float(self.material_model.volume_fraction), float(self.move),
-                  _lib.ptr(out), _lib.ptr(self._oc_work), steps, _lib.stream())
-        self.last_oc_steps = steps.value
+                  _lib.f64(out, n), _lib.f64(self._oc_work), None, _lib.stream())
         return out
 
+    @property
+    def last_oc_steps(self):
+        """Bisection steps of the last update_design (reads the device state: synchronises)."""
+        return int(self._oc_work[:5].view(torch.int32)[9].item())
+
     def _dot(self, a, b):
-        _lib.call("et_dot_f64", a.numel(), _lib.ptr(a), _lib.ptr(b), _lib.ptr(self._scalar[0:1]),
+        _lib.call("et_dot_f64", a.numel(), _lib.f64(a), _lib.f64(b, a.numel()), _lib.f64(self._scalar[0:1]),
                   _lib.stream())
         return float(self._scalar[0].item())
 
     def _rms_change(self, a, b):
-        _lib.call("et_sqdist_f64", a.numel(), _lib.ptr(a), _lib.ptr(b), _lib.ptr(self._scalar[1:2]),
+        _lib.call("et_sqdist_f64", a.numel(), _lib.f64(a), _lib.f64(b, a.numel()), _lib.f64(self._scalar[1:2]),
                   _lib.stream())
         return float(np.sqrt(self._scalar[1].item() / a.numel()))
 
@@ -137,8 +172,11 @@ class TopOpt:
         U, residual = self.solver.solve(self.f, use_last=self.reuse_sol)
         compliance = self._dot(self.f, U)
         pen = mm.penalty_schedule(mm.penalty, iteration) if mm.penalty_schedule else mm.penalty
-        dr = self.kernel.sensitivity(U, rho_, pen, mm.void)
-        dr = self.filter_kernel._rmatvec(dr)
+        if hasattr(self.filter_kernel, "simp_adjoint"):
+            # sensitivity and filter transpose in one kernel
+            dr = self.filter_kernel.simp_adjoint(self.kernel, U, rho_, pen, mm.void)
+        else:
+            dr = self.filter_kernel._rmatvec(self.kernel.sensitivity(U, rho_, pen, mm.void))
         return compliance, dr.reshape(-1, 1), residual
 
     def optimize(self, save_comp_history=False, save_change_history=False, save_rho_history=False,
@@ -189,11 +227,6 @@ class TopOpt:
         if save_rho_history:
             hist["rho_history"] = rho_history
         return rho, flag, hist
-
-
-def ctypes_int():
-    import ctypes
-    return ctypes.c_int(0)
 
 
 CuTopOpt = TopOpt

@@ -6,7 +6,7 @@ restricted to the structured GPU path:
     CuStructuredMesh2D / CuStructuredMesh3D     geom
     CuStructuredFilter2D / CuStructuredFilter3D geom
     CuStructuredStiffnessKernel                 kernels
-    CuMultiGrid                                 solvers
+    CuMultiGrid, CuCG, CuGMRES                  solvers
     SingleMaterial                              MaterialModels
     CuTopOpt                                    TopOpt
     auto_stiffness                              physics
@@ -23,9 +23,10 @@ from .kernels import CuStructuredStiffnessKernel, StiffnessKernel, StructuredSti
 from .MaterialModels import MaterialModel, SingleMaterial
 from .physics import (auto_stiffness, hexahedron_element_stiffness,
                       quadrilateral_element_stiffness)
-from .solvers import CuMultiGrid, MultiGrid, Solver
+from .solvers import CG, GMRES, CuCG, CuGMRES, CuMultiGrid, MultiGrid, Solver
 from .TopOpt import CuTopOpt, TopOpt
 from .autograd import ComplianceFunction, compliance, energy_compliance
+from .batch import BatchTopOpt, gmg_levels, random_problem
 
 _lib.load(require_gpu=False)
 
@@ -35,4 +36,5 @@ __all__ = ["CuStructuredMesh2D", "CuStructuredMesh3D", "CuStructuredFilter2D",
            "StructuredFilter2D", "StructuredFilter3D", "StructuredStiffnessKernel", "MultiGrid",
            "TopOpt", "FilterKernel", "StiffnessKernel", "MaterialModel", "Solver",
            "quadrilateral_element_stiffness", "hexahedron_element_stiffness", "compliance",
-           "ComplianceFunction", "energy_compliance"]
+           "ComplianceFunction", "energy_compliance", "CuCG", "CuGMRES", "CG", "GMRES", "BatchTopOpt",
+           "gmg_levels", "random_problem"]

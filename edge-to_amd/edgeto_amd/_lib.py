@@ -45,6 +45,7 @@ _SIGS = {
     "et_mg_coarse_solve": [_P, _P, _P],
     "et_tune_get": [_I],
     "et_generate_elements": [_I, _IP, _P, _P],
+    "et_k0_structure": [_I, ctypes.POINTER(ctypes.c_double), _IP],
     "et_matvec_f64": [_I, _IP, _P, _P, _P, _P, _P, _P],
     "et_matvec_f32": [_I, _IP, _P, _P, _P, _P, _P, _P],
     "et_matvec_f32_mfma": [_I, _IP, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P, _P, _P],
@@ -56,9 +57,11 @@ _SIGS = {
     "et_elem_energy_f64": [_I, _IP, _P, _P, _P, _P],
     "et_sensitivity_f64": [_I, _IP, _P, _P, _P, _D, _D, _P, _P],
     "et_simp_forward_f64": [_I64, _P, _D, _D, _I, _P, _P],
+    "et_fea_integrals_f64": [_I, _IP, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "et_filter_normalizer_f64": [_I, _IP, _I, _P, _P, _P, _P],
     "et_filter_fwd_f64": [_I, _IP, _I, _P, _P, _P, _P, _P],
     "et_filter_adj_f64": [_I, _IP, _I, _P, _P, _P, _P, _P, _P],
+    "et_simp_adjoint_f64": [_I, _IP, _P, _P, _P, _D, _D, _I, _P, _P, _P, _P, _P],
     "et_restrict_f64": [_I, _IP, _I, _P, _P, _P],
     "et_prolong_f64": [_I, _IP, _I, _P, _P, _P],
     "et_stencil_from_operator_f64": [_I, _IP, _P, _P, _P, _P, _P],
@@ -68,7 +71,16 @@ _SIGS = {
     "et_dot_f64": [_I64, _P, _P, _P, _P],
     "et_sqdist_f64": [_I64, _P, _P, _P, _P],
     "et_oc_work_size": [_I64],
+    "et_krylov_work_size": [_I64, _I],
+    "et_cg_f64": [_I, _IP, _P, _P, _P, _P, _P, _D, _I, _P, _I64, ctypes.POINTER(ctypes.c_int),
+                  ctypes.POINTER(ctypes.c_double), _P],
+    "et_gmres_f64": [_I, _IP, _P, _P, _P, _P, _P, _D, _I, _I, _P, _I64, ctypes.POINTER(ctypes.c_int),
+                     ctypes.POINTER(ctypes.c_double), _P],
     "et_oc_update_f64": [_I64, _P, _P, _D, _D, _D, _D, _P, _P, ctypes.POINTER(ctypes.c_int), _P],
+    "et_batch_create": [ctypes.POINTER(_P), _I, _P, _I],
+    "et_batch_run": [_P, _I, _I, ctypes.POINTER(ctypes.c_double)],
+    "et_batch_result": [_P, _I, _P, _P, _P, _P],
+    "et_batch_destroy": [_P],
     "et_mg_create": [ctypes.POINTER(_P), ctypes.POINTER(MGParams), ctypes.POINTER(ctypes.c_double), _P],
     "et_mg_destroy": [_P],
     "et_mg_solve": [_P, _P, _P, _P, _P, _D, _I, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)],
@@ -76,6 +88,8 @@ _SIGS = {
     "et_mg_profile": [_P, _I],
     "et_mg_profile_read": [_P, _P, _P],
 }
+
+_I64_RET = ("et_oc_work_size", "et_krylov_work_size")
 
 PROF_KINDS = ["l0_jacobi", "l0_residual", "l0_jacobi_add", "l0_scale", "matvec64", "coarse_ops",
               "transfer", "coarse_solve", "pcg_vector", "setup", "l0_jacobi0"]
@@ -95,7 +109,7 @@ def load(require_gpu=True):
     for name, args in _SIGS.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        fn.restype = ctypes.c_int64 if name == "et_oc_work_size" else ctypes.c_int
+        fn.restype = ctypes.c_int64 if name in _I64_RET else ctypes.c_int
     lib.et_last_error.restype = ctypes.c_char_p
     lib.et_last_error.argtypes = []
     lib.et_version.restype = ctypes.c_char_p
@@ -124,15 +138,41 @@ def stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def ptr(t):
-    """Device pointer of a contiguous CUDA tensor (or None -> NULL)."""
+def ptr(t, dtype=None, numel=None):
+    """Device pointer of a contiguous CUDA tensor (or None -> NULL).
+
+    dtype / numel, when given, must match: the kernels read and write exactly
+    numel values of that type, so a mismatch would be an out-of-bounds access."""
     if t is None:
         return None
-    if not t.is_cuda:
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
         raise ValueError("expected a device tensor")
     if not t.is_contiguous():
         raise ValueError("expected a contiguous tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"expected a {dtype} tensor, got {t.dtype}")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"expected {numel} values, got {t.numel()}")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def f64(t, numel=None):
+    return ptr(t, torch.float64, numel)
+
+
+def f32(t, numel=None):
+    return ptr(t, torch.float32, numel)
+
+
+def i32(t, numel=None):
+    return ptr(t, torch.int32, numel)
+
+
+def u8(t, numel=None):
+    """bool or uint8 tensor (one byte per entry)."""
+    if t is not None and isinstance(t, torch.Tensor) and t.dtype not in (torch.bool, torch.uint8):
+        raise ValueError(f"expected a bool/uint8 tensor, got {t.dtype}")
+    return ptr(t, None, numel)
 
 
 def nel_arg(nel):

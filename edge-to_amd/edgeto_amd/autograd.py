@@ -4,9 +4,10 @@ Forward (TopOpt/_cuda.py:282-291): x~ = H x, E = clip(x~^p, void, 1), solve
 A_hat(E) u = f with the GMG-PCG solver, c = f . u.
 Backward: the problem is self-adjoint, so the adjoint state is u itself and
     dc/dx = H^T [ p clip(x~, void, 1)^(p-1) * (-u_e^T K0 u_e) ]
-(TopOpt/_cuda.py:293-300), evaluated by two library kernels: the fused SIMP
-sensitivity (element energy times the SIMP factor) and the exact filter
-transpose.  No autograd tape is recorded through the solver.
+(TopOpt/_cuda.py:293-300), evaluated by one library kernel (et_simp_adjoint_f64:
+element energy times the SIMP factor, divided by the filter normaliser while
+staged in LDS, then the transpose stencil).  No autograd tape is recorded
+through the solver.
 """
 import torch
 
@@ -18,13 +19,16 @@ class ComplianceFunction(torch.autograd.Function):
     def forward(ctx, x, topopt, iteration=0):
         tp = topopt
         mm = tp.material_model
-        xt = tp.filter_kernel.dot(x.detach().contiguous())
+        ctx.in_dtype = x.dtype
+        # the path computes in FP64; other input dtypes are cast (and the gradient cast back)
+        xt = tp.filter_kernel.dot(x.detach().to(torch.float64).contiguous())
         E = mm(xt, iteration)
         tp.kernel.set_rho(E)
         U, residual = tp.solver.solve(tp.f, use_last=tp.reuse_sol)
         U = U.clone()  # the solver keeps its warm-start buffer
         out = torch.zeros(1, dtype=torch.float64, device=x.device)
-        _lib.call("et_dot_f64", tp.f.numel(), _lib.ptr(tp.f), _lib.ptr(U), _lib.ptr(out), _lib.stream())
+        _lib.call("et_dot_f64", tp.f.numel(), _lib.f64(tp.f), _lib.f64(U, tp.f.numel()), _lib.f64(out),
+                  _lib.stream())
         ctx.save_for_backward(U, xt)
         ctx.tp = tp
         ctx.iteration = iteration
@@ -37,9 +41,11 @@ class ComplianceFunction(torch.autograd.Function):
         tp = ctx.tp
         mm = tp.material_model
         pen = mm.penalty_schedule(mm.penalty, ctx.iteration) if mm.penalty_schedule else mm.penalty
-        dr = tp.kernel.sensitivity(U, xt, pen, mm.void)
-        dx = tp.filter_kernel._rmatvec(dr)
-        return dx * grad_c, None, None
+        if hasattr(tp.filter_kernel, "simp_adjoint"):
+            dx = tp.filter_kernel.simp_adjoint(tp.kernel, U, xt, pen, mm.void)
+        else:
+            dx = tp.filter_kernel._rmatvec(tp.kernel.sensitivity(U, xt, pen, mm.void))
+        return (dx * grad_c).to(ctx.in_dtype), None, None
 
 
 def compliance(x, topopt, iteration=0):
@@ -53,6 +59,7 @@ def energy_compliance(u, topopt):
     tp = topopt
     au = tp.kernel.dot(u)
     out = torch.zeros(2, dtype=torch.float64, device=u.device)
-    _lib.call("et_dot_f64", u.numel(), _lib.ptr(tp.f), _lib.ptr(u), _lib.ptr(out[0:1]), _lib.stream())
-    _lib.call("et_dot_f64", u.numel(), _lib.ptr(u), _lib.ptr(au), _lib.ptr(out[1:2]), _lib.stream())
+    n = u.numel()
+    _lib.call("et_dot_f64", n, _lib.f64(tp.f, n), _lib.f64(u), _lib.f64(out[0:1]), _lib.stream())
+    _lib.call("et_dot_f64", n, _lib.f64(u), _lib.f64(au, n), _lib.f64(out[1:2]), _lib.stream())
     return float(2.0 * out[0].item() - out[1].item())

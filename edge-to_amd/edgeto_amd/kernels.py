@@ -105,14 +105,16 @@ class StructuredStiffnessKernel(StiffnessKernel):
 
     # --- operator ---
     def _cons_ptr(self):
-        return _lib.ptr(self.constraints) if self.has_con else None
+        return _lib.u8(self.constraints, self.shape[0]) if self.has_con else None
 
     def _matvec(self, rho, vec):
         if self.mat_vec is None:
             self.mat_vec = torch.empty_like(vec)
         vec = vec.contiguous()  # keep the (possibly new) tensor referenced across the call
-        _lib.call("et_matvec_f64", self._dim, _lib.nel_arg(self._nel), _lib.ptr(self.K_single),
-                  _lib.ptr(rho), self._cons_ptr(), _lib.ptr(vec), _lib.ptr(self.mat_vec),
+        n = self.shape[0]
+        _lib.call("et_matvec_f64", self._dim, _lib.nel_arg(self._nel), _lib.f64(self.K_single),
+                  _lib.f64(rho, self.elements.shape[0]), self._cons_ptr(), _lib.f64(vec, n),
+                  _lib.f64(self.mat_vec, n),
                   _lib.stream())
         return self.mat_vec
 
@@ -136,25 +138,27 @@ class StructuredStiffnessKernel(StiffnessKernel):
         r = self.rho if rho is None else rho.contiguous()
         if self.diag is None:
             self.diag = torch.empty(self.shape[0], dtype=torch.float64, device=self.constraints.device)
-        _lib.call("et_diagonal_f64", self._dim, _lib.nel_arg(self._nel), _lib.ptr(self.K_single),
-                  _lib.ptr(r), self._cons_ptr(), _lib.ptr(self.diag), _lib.stream())
+        _lib.call("et_diagonal_f64", self._dim, _lib.nel_arg(self._nel), _lib.f64(self.K_single),
+                  _lib.f64(r, self.elements.shape[0]), self._cons_ptr(), _lib.f64(self.diag, self.shape[0]),
+                  _lib.stream())
         return self.diag
 
     def process_grad(self, U):
         if self.dk is None:
             self.dk = torch.empty(self.elements.shape[0], dtype=torch.float64, device=U.device)
         U = U.contiguous()
-        _lib.call("et_elem_energy_f64", self._dim, _lib.nel_arg(self._nel), _lib.ptr(self.K_single),
-                  _lib.ptr(U), _lib.ptr(self.dk), _lib.stream())
+        _lib.call("et_elem_energy_f64", self._dim, _lib.nel_arg(self._nel), _lib.f64(self.K_single),
+                  _lib.f64(U, self.shape[0]), _lib.f64(self.dk, self.elements.shape[0]), _lib.stream())
         return self.dk
 
     def sensitivity(self, U, xt, penal, void, out=None):
         """p * clip(xt, void, 1)^(p-1) * (-u_e^T K0 u_e) in one kernel."""
         out = torch.empty(self.elements.shape[0], dtype=torch.float64, device=U.device) if out is None else out
         U, xt = U.contiguous(), xt.contiguous()
-        _lib.call("et_sensitivity_f64", self._dim, _lib.nel_arg(self._nel), _lib.ptr(self.K_single),
-                  _lib.ptr(U), _lib.ptr(xt), float(penal), float(void),
-                  _lib.ptr(out), _lib.stream())
+        ne = self.elements.shape[0]
+        _lib.call("et_sensitivity_f64", self._dim, _lib.nel_arg(self._nel), _lib.f64(self.K_single),
+                  _lib.f64(U, self.shape[0]), _lib.f64(xt, ne), float(penal), float(void),
+                  _lib.f64(out, ne), _lib.stream())
         return out
 
 

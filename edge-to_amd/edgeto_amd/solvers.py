@@ -12,11 +12,14 @@ inside the FP64 outer loop.  The reference's 'W' default behaves as a
 V-cycle (it tests for a lowercase 'w', :165); the same rule applies here.
 """
 import ctypes
+import logging
 
 import numpy as np
 import torch
 
 from . import _lib
+
+logger = logging.getLogger(__name__)
 
 
 class Solver:
@@ -113,9 +116,10 @@ class MultiGrid(Solver):
         rhs = rhs.contiguous()
         it = ctypes.c_int(0)
         res = ctypes.c_double(0.0)
-        status = _lib.lib().et_mg_solve(self._handle, _lib.ptr(self.kernel.rho),
-                                        _lib.ptr(self.kernel.constraints), _lib.ptr(rhs),
-                                        _lib.ptr(x), float(self.tol), int(self.maxiter),
+        n = self.kernel.shape[0]
+        status = _lib.lib().et_mg_solve(self._handle, _lib.f64(self.kernel.rho, self.mesh.n_el),
+                                        _lib.u8(self.kernel.constraints, n), _lib.f64(rhs, n),
+                                        _lib.f64(x, n), float(self.tol), int(self.maxiter),
                                         ctypes.byref(it), ctypes.byref(res))
         if status != _lib.ET_OK and (self.strict or status != _lib.ET_ENOCONV):
             _lib.check("et_mg_solve", status)
@@ -128,3 +132,78 @@ class MultiGrid(Solver):
 
 
 CuMultiGrid = MultiGrid
+
+
+class _Krylov(Solver):
+    """Shared part of CG / GMRES (solvers/_gpu.py:11-107): the reference hands the stiffness
+    operator to cupyx.scipy.sparse.linalg.cg / gmres; here the whole iteration runs in the
+    library (et_cg_f64 / et_gmres_f64) on the matrix-free structured operator."""
+
+    _restart = 0
+
+    def __init__(self, kernel, maxiter=1000, tol=1e-5, matrix_free=None):
+        super().__init__()
+        if matrix_free is False:
+            logger.warning("matrix_free=False: structured kernels are always applied matrix-free "
+                           "(no assembled matrix is built); the result is the same operator.")
+        self.kernel = kernel
+        self.last_x0 = None
+        self.tol = tol
+        self.maxiter = maxiter
+        self.matrix_free = True
+        self.last_iterations = 0
+        self._work = None
+
+    def _call(self, name, rhs, x):
+        k = self.kernel
+        n = k.shape[0]
+        size = int(_lib.lib().et_krylov_work_size(n, self._restart))
+        if self._work is None or self._work.numel() * 8 < size:
+            self._work = torch.empty(size // 8 + 1, dtype=torch.float64, device=rhs.device)
+        it, res = ctypes.c_int(0), ctypes.c_double(0.0)
+        extra = (int(self._restart),) if self._restart else ()
+        status = getattr(_lib.lib(), name)(
+            len(k._nel), _lib.nel_arg(k._nel), _lib.f64(k.K_single), _lib.f64(k.rho, k.elements.shape[0]),
+            k._cons_ptr(), _lib.f64(rhs, n), _lib.f64(x, n), float(self.tol), *extra, int(self.maxiter),
+            _lib.f64(self._work), self._work.numel() * 8, ctypes.byref(it), ctypes.byref(res), _lib.stream())
+        # maxiter reached above the tolerance is not an error (cupyx reports it in `info` only)
+        if status != _lib.ET_OK and status != _lib.ET_ENOCONV:
+            _lib.check(name, status)
+        self.last_iterations = it.value
+        return res.value
+
+    def solve(self, rhs, rho=None, use_last=True):
+        if not (self.kernel.has_rho or rho is not None):
+            raise ValueError("Solver requires a density vector to be passed or set on the kernel.")
+        if rho is not None:
+            self.kernel.set_rho(rho)
+        if not (isinstance(rhs, torch.Tensor) and rhs.is_cuda and rhs.dtype == torch.float64):
+            raise ValueError("rhs must be a float64 device tensor")
+        rhs = rhs.contiguous()
+        if use_last and self.last_x0 is not None:
+            x = self.last_x0.clone()
+        else:
+            x = torch.zeros_like(rhs)
+        residual = self._call(self._name, rhs, x)
+        if use_last:
+            self.last_x0 = x
+        return x, residual
+
+    def reset(self):
+        self.last_x0 = None
+
+
+class CG(_Krylov):
+    """Conjugate gradients without preconditioning (solvers/_gpu.py:11-56)."""
+    _name = "et_cg_f64"
+
+
+class GMRES(_Krylov):
+    """Restarted GMRES(20) without preconditioning (solvers/_gpu.py:58-103), cupyx's
+    default restart length."""
+    _name = "et_gmres_f64"
+    _restart = 20
+
+
+CuCG = CG
+CuGMRES = GMRES

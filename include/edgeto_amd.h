@@ -102,6 +102,23 @@ enum {
 int et_tune_set(int param, int value);
 int et_tune_get(int param);
 
+/* Structure of an element matrix (host memory, FP64, row-major) that the fast
+ * 3-D kernels rely on.  *flags receives the OR of
+ *   ET_K0_SYMMETRIC  K0 = K0^T;
+ *   ET_K0_REFLECT    3-D: with corners in bit order (x + 2y + 4z), every 3x3 block
+ *                    K0[a][b] = R_a K0[0][a^b] R_a, R_a = diag((-1)^a_x, (-1)^a_y, (-1)^a_z)
+ *                    (isotropic or axis-aligned orthotropic material on any brick);
+ *   ET_K0_WHT        3-D: only the couplings the transformed-basis operator keeps are
+ *                    non-zero in the corner Walsh-Hadamard basis;
+ *   ET_K0_LEVEL1     3-D: the matrix-free first coarse level covers the matrix;
+ * each within 1e-12 * max|K0|.  The reference takes any element matrix through its
+ * stiffness= callable (geom/_mesh.py:113-135); the library routes a 3-D matrix without
+ * SYMMETRIC | REFLECT | WHT to the general node-gather kernels (et_matvec_*,
+ * et_residual_*, et_jacobi_*, et_mg_create) and one without LEVEL1 to a stored
+ * first coarse level.  No counterpart in the reference. */
+enum { ET_K0_SYMMETRIC = 1, ET_K0_REFLECT = 2, ET_K0_WHT = 4, ET_K0_LEVEL1 = 8 };
+int et_k0_structure(int dim, const double* K0_host, int* flags);
+
 /* ---------------------------------------------------------------- mesh --- */
 
 /* Element connectivity (nel x 4|8, int32), bit-exact to
@@ -113,7 +130,9 @@ int et_generate_elements(int dim, const int* nel, int32_t* elements, hipStream_t
 /* y = A_hat u with A_hat = M K(E) M + I_c: constrained rows copy u, constrained
  * columns are skipped.  Replaces mat_vec_node_basis_parallel_wcon_cuda_kernel
  * (core/_templated_cuda.cu:782-834) behind StructuredStiffnessKernel._matvec
- * (kernels/_cuda.py:173-204).  cons may be NULL (no constraints, :137-178). */
+ * (kernels/_cuda.py:173-204).  cons may be NULL (no constraints, :137-178).
+ * 3-D grids (also et_residual_*, et_jacobi_*): K0 is read back to the host to
+ * check its structure (et_k0_structure), which synchronises `stream`. */
 int et_matvec_f64(int dim, const int* nel, const double* K0, const double* E,
                   const uint8_t* cons, const double* u, double* y, hipStream_t stream);
 int et_matvec_f32(int dim, const int* nel, const float* K0, const float* E,
@@ -163,6 +182,18 @@ int et_sensitivity_f64(int dim, const int* nel, const double* K0, const double* 
                        const double* xt, double penal, double void_, double* out,
                        hipStream_t stream);
 
+/* Element post-processing of a displacement U (n_dof, constrained DOFs included):
+ *   strain_e = B u_e, stress_e = D strain_e, energy_e = 1/2 E_e u_e^T K0 u_e, and the
+ *   von Mises stress of stress_e (plane stress in 2-D).
+ * D: NB x NB and B: NB x (4|8)*dim row-major device arrays (NB = 3 in 2-D, 6 in 3-D;
+ * strain order xx, yy[, zz], xy[, yz, zx] with engineering shears).  Outputs:
+ * strain, stress (n_elem x NB), energy (n_elem), von_mises (n_elem; may be NULL).
+ * Replaces FEA_Integrals_node_basis_parallel_cuda_kernel (core/_templated_cuda.cu:1681-1719)
+ * and the von Mises step of TopOpt.FEA_integrals (TopOpt/_cuda.py:78-129). */
+int et_fea_integrals_f64(int dim, const int* nel, const double* K0, const double* D, const double* B,
+                         const double* E, const double* U, double* strain, double* stress,
+                         double* energy, double* von_mises, hipStream_t stream);
+
 /* E = clip(xt^p, void, 1) (plain: clip(xt, void, 1)), MaterialModels.py:47-60. */
 int et_simp_forward_f64(int64_t n, const double* xt, double penal, double void_, int plain,
                         double* E, hipStream_t stream);
@@ -180,6 +211,16 @@ int et_filter_fwd_f64(int dim, const int* nel, int n_off, const int32_t* offsets
 int et_filter_adj_f64(int dim, const int* nel, int n_off, const int32_t* offsets,
                       const double* w, const double* norm, const double* v, double* y,
                       hipStream_t stream);
+
+/* The SIMP adjoint fused with the filter transpose, one kernel:
+ *   out = H^T s,  s_e = p * clip(xt_e, void, 1)^(p-1) * (-u_e^T K0 u_e),
+ * i.e. et_sensitivity_f64 followed by et_filter_adj_f64 (CuTopOpt.solve_and_process,
+ * TopOpt/_cuda.py:293-300) with s_e / N_e formed once per element while it is staged
+ * in LDS.  U: n_dof displacement (constrained DOFs included); xt: filtered design;
+ * offsets, w, norm: the filter's table and normaliser. */
+int et_simp_adjoint_f64(int dim, const int* nel, const double* K0, const double* U, const double* xt,
+                        double penal, double void_, int n_off, const int32_t* offsets, const double* w,
+                        const double* norm, double* out, hipStream_t stream);
 
 /* ------------------------------------------------- multigrid transfers --- */
 /* Restriction fine -> coarse (nel = fine element counts): restriction_{2d,3d}_kernel
@@ -221,13 +262,36 @@ int et_dot_f64(int64_t n, const double* a, const double* b, double* out, hipStre
  * (TopOpt/_cuda.py:350) is sqrt(*out / n). */
 int et_sqdist_f64(int64_t n, const double* a, const double* b, double* out, hipStream_t stream);
 
+/* ------------------------------------------------------ Krylov solvers --- */
+/* Unpreconditioned solvers of A_hat x = b on the matrix-free operator (the
+ * iterations CuCG / CuGMRES run through cupyx cg / gmres, solvers/_gpu.py:11-107).
+ * x holds the initial guess on entry.  atol = tol |b|.  work: device buffer of at
+ * least et_krylov_work_size(n_dof, restart) bytes (restart 0 for CG), given in
+ * work_bytes.  Synchronise `stream`.  *iters: CG iterations, or GMRES inner steps
+ * (restart per cycle); *residual: |b - A x| / |b| of the returned x.  Status
+ * ET_ENOCONV when maxiter ends the loop above the tolerance.
+ * et_cg_f64: rho = r.r, p = r + (rho/rho_old) p, alpha = rho / p.A p, stop on |r| <= atol.
+ * et_gmres_f64: cycles of `restart` Arnoldi steps (one classical Gram-Schmidt pass),
+ * least squares by Givens rotations, convergence tested once per cycle. */
+int64_t et_krylov_work_size(int64_t n_dof, int restart);
+int et_cg_f64(int dim, const int* nel, const double* K0, const double* E, const uint8_t* cons,
+              const double* b, double* x, double tol, int maxiter, void* work, int64_t work_bytes,
+              int* iters, double* residual, hipStream_t stream);
+int et_gmres_f64(int dim, const int* nel, const double* K0, const double* E, const uint8_t* cons,
+                 const double* b, double* x, double tol, int restart, int maxiter, void* work,
+                 int64_t work_bytes, int* iters, double* residual, hipStream_t stream);
+
 /* ------------------------------------------------- optimality criteria --- */
 /* One OC design update (TopOpt/_cuda.py:242-280 with SingleMaterial.__ocP__ /
  * __con__, MaterialModels.py:75-90): bisection on lambda in [1e-9, 1e9] until
  * (l2-l1)/(l2+l1) <= 1e-4; the returned design is the one at the last midpoint.
  * A = element measure (structured: one value), vol = A * n, vf = volume fraction.
- * Synchronises `stream` (the bisection count is data-dependent); *steps receives
- * the number of bisection steps.  work: device buffer of et_oc_work_size(n) bytes. */
+ * Runs entirely on the device (a fixed sequence of kernels, no host round trip):
+ * with steps == NULL the call is asynchronous; with steps != NULL it synchronises
+ * `stream` and *steps receives the number of bisection steps.  work: device buffer
+ * of et_oc_work_size(n) bytes, 8-byte aligned; it begins with the bisection state
+ * {double l1, l2, lmid, sabs; int32 done, steps} (40 bytes), so the step count
+ * can also be read from work + 36 once the stream has passed the update. */
 int64_t et_oc_work_size(int64_t n);
 int et_oc_update_f64(int64_t n, const double* x, const double* dc, double A, double vol,
                      double vf, double move, double* x_new, void* work, int* steps,
@@ -304,6 +368,44 @@ enum {
 };
 int et_mg_profile(et_mg* handle, int enable);
 int et_mg_profile_read(et_mg* handle, double* ms, int64_t* launches);
+
+/* ------------------------------------------------------ batched engine --- */
+/* Many independent structured SIMP problems optimised concurrently on one GPU
+ * (the batched workload: each problem is CuTopOpt.optimize, TopOpt/_cuda.py:304-382,
+ * with its own mesh, BCs, loads, filter radius, volume fraction and GMG depth).
+ * The engine copies every array at creation and keeps each problem's state and its
+ * GMG-PCG solver resident; et_batch_run runs, for every problem, the initial solve and
+ * n_iter OC iterations, n_workers host threads each driving one problem at a time on
+ * that problem's stream (largest problems first).  Between two solves a problem issues
+ * one captured graph (compliance, fused SIMP adjoint, OC update, design change, filter,
+ * SIMP map), captured on its first iteration and replayed.  Synchronous.  No
+ * counterpart in the reference (it optimises one problem per process). */
+typedef struct et_batch et_batch;
+typedef struct et_batch_problem {
+  int dim;
+  int nel[3];
+  const double* K0;        /* host: element matrix ((4|8) dim)^2, row-major */
+  const uint8_t* cons;     /* host: n_dof constraint bytes */
+  const double* f;         /* host: n_dof loads */
+  const double* x0;        /* host: n_elem initial design (clipped to [void, 1]) */
+  int n_off;               /* filter taps */
+  const int32_t* offsets;  /* host: n_off x dim */
+  const double* weights;   /* host: n_off normalised hat weights */
+  double A;                /* element measure */
+  double volfrac, penal, void_, move, tol;
+  int maxiter, n_level, n_smooth;
+  /* transformed-basis tiles of this problem's sweeps (0 = library default) */
+  int tile_threads_f32, tile_threads_f64, tile_slices_f32, tile_slices_f64;
+} et_batch_problem;
+int et_batch_create(et_batch** handle, int n, const et_batch_problem* problems, int precision);
+/* *wall_ms: host wall time of the whole run. */
+int et_batch_run(et_batch* handle, int n_iter, int n_workers, double* wall_ms);
+/* Results of problem i of the last run: compliance (n_iter + 1), PCG iterations
+ * (n_iter + 1), wall time of each OC iteration in ms (n_iter), final design (n_elem,
+ * host).  Any pointer may be NULL. */
+int et_batch_result(et_batch* handle, int i, double* compliance, int* pcg_iterations,
+                    double* iteration_ms, double* x_host);
+int et_batch_destroy(et_batch* handle);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@ rows return the input entry, constrained columns are skipped.
   matvec (no cons)  core/_templated_cuda.cu:137-178, core/_ops.py:212-244
   diagonal          core/_templated_cuda.cu:1068-1113, core/_ops.py:45-75
   process_dk        core/_templated_cuda.cu:180-194, core/_ops.py:5-13
+  fea_integrals     TopOpt/_cuda.py:78-129, core/_templated_cuda.cu:1681-1719
 
 Pinned in tests against a SciPy COO assembly of the same operator (built from
 the element list independently of the gather) and, on tiny meshes, against a
@@ -103,3 +104,25 @@ def process_dk(K0, elements, U):
     dof = K0.shape[0] // elements.shape[1]
     ue = np.asarray(U, dtype=np.float64)[dof_map(elements, dof)]
     return -np.einsum("ei,ij,ej->e", ue, K0, ue)
+
+
+def fea_integrals(K0, D, B, elements, E, U):
+    """Per-element post-processing of a solved displacement (TopOpt/_cuda.py:78-129 with
+    FEA_Integrals_node_basis_parallel_cuda_kernel, core/_templated_cuda.cu:1681-1719):
+      strain_e = B u_e (B: the element's Gauss-averaged strain-displacement matrix),
+      stress_e = D strain_e (not scaled by the density),
+      energy_e = 1/2 u_e^T (E_e K0) u_e,
+      von Mises from the stress (2-D plane stress / 3-D, TopOpt/_cuda.py:121-124).
+    u_e is the full displacement (constrained DOFs included)."""
+    dof = K0.shape[0] // elements.shape[1]
+    ue = np.asarray(U, dtype=np.float64)[dof_map(elements, dof)]
+    strain = ue @ B.T
+    stress = strain @ D.T
+    energy = 0.5 * np.asarray(E, dtype=np.float64) * np.einsum("ei,ij,ej->e", ue, K0, ue)
+    s = stress
+    if B.shape[0] == 3:
+        vm = np.sqrt(s[:, 0] ** 2 + s[:, 1] ** 2 - s[:, 0] * s[:, 1] + 3 * s[:, 2] ** 2)
+    else:
+        vm = np.sqrt(0.5 * ((s[:, 0] - s[:, 1]) ** 2 + (s[:, 1] - s[:, 2]) ** 2 + (s[:, 2] - s[:, 0]) ** 2
+                            + 6 * (s[:, 3] ** 2 + s[:, 4] ** 2 + s[:, 5] ** 2)))
+    return strain, stress, energy, vm

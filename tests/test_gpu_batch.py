@@ -1,0 +1,78 @@
+"""The batched engine (BatchTopOpt / et_batch_*): every problem of a seeded batch of small
+2-D and 3-D problems, optimised concurrently by four worker threads, against its own
+single-problem CuTopOpt run (the same kernels one problem at a time): compliance history,
+PCG iteration counts and final design to 1e-10."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _small_problem(i):
+    """The seeded batch's recipe (edgeto_amd.batch.random_problem) at test sizes."""
+    rng = np.random.default_rng(500 + i)
+    dim = 2 if i % 2 == 0 else 3
+    nel = (int(rng.integers(24, 61)), int(rng.integers(12, 33))) if dim == 2 else \
+        (int(rng.integers(8, 17)) * 2,) * 1 + (int(rng.integers(6, 11)),) * 2
+    side = int(rng.integers(0, 4))
+    nn = [n + 1 for n in nel]
+    if dim == 2:
+        j, i_ = (a.ravel() for a in np.indices((nn[1], nn[0])))
+        k = np.zeros_like(i_)
+        node = j * nn[0] + i_
+    else:
+        j, i_, k = (a.ravel() for a in np.indices((nn[1], nn[0], nn[2])))
+        node = (j * nn[0] + i_) * nn[2] + k
+    on_side = [i_ == 0, i_ == nel[0], j == 0, j == nel[1]]
+    boundary = on_side[0] | on_side[1] | on_side[2] | on_side[3]
+    if dim == 3:
+        boundary |= (k == 0) | (k == nel[2])
+    n_loads = int(rng.integers(1, 5))
+    loads = rng.choice(node[boundary & ~on_side[side]], size=n_loads, replace=False)
+    dirs = rng.standard_normal((n_loads, dim))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    return dict(dim=dim, nel=nel, clamp=node[on_side[side]], loads=loads, dirs=dirs,
+                vf=float(rng.uniform(0.2, 0.6)), rmin=float(rng.uniform(1.2, 3.0)), penal=3.0)
+
+
+def _single(ET, pb, n_iter, n_level):
+    dim, nel = pb["dim"], pb["nel"]
+    stiff = lambda x0: ET.auto_stiffness(x0, 1.0, 0.3)
+    if dim == 2:
+        mesh = ET.CuStructuredMesh2D(*nel, *nel, stiffness=stiff)
+        filt = ET.CuStructuredFilter2D(mesh, pb["rmin"])
+    else:
+        mesh = ET.CuStructuredMesh3D(*nel, *nel, stiffness=stiff)
+        filt = ET.CuStructuredFilter3D(mesh, pb["rmin"])
+    kernel = ET.CuStructuredStiffnessKernel(mesh)
+    mat = ET.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=pb["vf"])
+    solver = ET.CuMultiGrid(mesh, kernel, tol=1e-8, n_smooth=2, n_level=n_level, precision="mixed")
+    tp = ET.CuTopOpt(mesh, mat, kernel, solver, filt, max_iter=n_iter, ch_tol=0, fun_tol=0, verbose=False)
+    tp.add_BC_nodal(pb["clamp"], np.ones((1, dim), dtype=np.int64))
+    tp.add_F_nodal(pb["loads"], pb["dirs"])
+    x, _, hist = tp.optimize(save_comp_history=True, rho_init=np.full(mesh.n_el, pb["vf"]))
+    comp = np.array([tp.iteration_log[0]["comp"]] + hist["comp_history"])
+    pcg = np.array([e["pcg"] for e in tp.iteration_log])
+    return comp, pcg, x.cpu().numpy()
+
+
+def test_batch_matches_single_problem_runs():
+    import edgeto_amd as ET
+    n_iter = 4
+    problems = [_small_problem(i) for i in range(8)]
+    batch = ET.BatchTopOpt(problems, precision="mixed", tiles={})
+    out = batch.run(n_iter, n_workers=4)
+    assert out["wall_ms"] > 0
+    for i, pb in enumerate(problems):
+        comp, pcg, x = _single(ET, pb, n_iter, batch.levels[i])
+        got = out["problems"][i]
+        rel = np.abs(got["compliance"] - comp) / np.abs(comp)
+        assert rel.max() <= 1e-10, (i, pb["nel"], rel.max())
+        assert np.array_equal(got["pcg"], pcg), (i, got["pcg"], pcg)
+        assert np.max(np.abs(batch.design(i) - x)) <= 1e-10
+        assert got["iter_ms"].shape == (n_iter,) and np.all(got["iter_ms"] > 0)
+    # a second run restarts every problem from its initial design: the same results
+    again = batch.run(n_iter, n_workers=2)
+    for i in range(len(problems)):
+        assert np.array_equal(again["problems"][i]["compliance"], out["problems"][i]["compliance"])

@@ -168,6 +168,53 @@ def test_diagonal_and_energy(nel):
     edof = mesh.dof_map(el[e:e + 1], len(nel))[0]
     assert np.isclose(dk[e], -u[edof] @ K0 @ u[edof], rtol=1e-13)
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fea_integrals_affine_field(dim):
+    """A displacement u(x) = G x (G arbitrary, brick elements of non-unit size) has the uniform
+    small strain eps = sym(G) in every element: strain = eps (engineering shears), stress =
+    C eps, element energy = 1/2 E_e V eps.C.eps, and von Mises of that stress; checked against
+    closed forms and the textbook von Mises cases (uniaxial: |s|, pure shear: sqrt(3)|t|)."""
+    rng = np.random.default_rng(21)
+    nel = (3, 2) if dim == 2 else (3, 2, 2)
+    size = (1.5, 0.7) if dim == 2 else (1.5, 0.7, 0.4)
+    if dim == 2:
+        el, nodes = mesh.elements_2d(*nel), mesh.nodes_2d(*nel, *[n * s for n, s in zip(nel, size)])
+        K0, C, B = physics.q4_stiffness(nodes[el[0]], 1.0, 0.3)
+        V = size[0] * size[1]
+    else:
+        el, nodes = mesh.elements_3d(*nel), mesh.nodes_3d(*nel, *[n * s for n, s in zip(nel, size)])
+        K0, C, B = physics.h8_stiffness(nodes[el[0]], 1.0, 0.3)
+        V = size[0] * size[1] * size[2]
+    G = rng.standard_normal((dim, dim))
+    U = (nodes @ G.T).ravel()
+    E = rng.uniform(0.2, 1.0, el.shape[0])
+    eps = 0.5 * (G + G.T)
+    if dim == 2:
+        ev = np.array([eps[0, 0], eps[1, 1], 2 * eps[0, 1]])
+    else:
+        ev = np.array([eps[0, 0], eps[1, 1], eps[2, 2], 2 * eps[0, 1], 2 * eps[1, 2], 2 * eps[2, 0]])
+    strain, stress, energy, vm = ops.fea_integrals(K0, C, B, el, E, U)
+    assert np.allclose(strain, ev[None], atol=1e-13)
+    assert np.allclose(stress, (C @ ev)[None], atol=1e-13)
+    assert np.allclose(energy, 0.5 * E * V * (ev @ C @ ev), rtol=1e-12)
+    # von Mises closed forms: affine fields whose stress is uniaxial or pure shear
+    if dim == 2:
+        uni, shear = np.array([2.5, 0.0, 0.0]), np.array([0.0, 0.0, 1.7])
+    else:
+        uni, shear = np.array([2.5, 0, 0, 0, 0, 0.0]), np.array([0, 0, 0, 0, 1.7, 0.0])
+    Dinv = np.linalg.inv(C)
+    for target, vm_expect in ((uni, 2.5), (shear, np.sqrt(3) * 1.7)):
+        e_t = Dinv @ target  # the strain with this stress
+        if dim == 2:
+            Gt = np.array([[e_t[0], e_t[2] / 2], [e_t[2] / 2, e_t[1]]])
+        else:
+            Gt = np.array([[e_t[0], e_t[3] / 2, e_t[5] / 2], [e_t[3] / 2, e_t[1], e_t[4] / 2],
+                           [e_t[5] / 2, e_t[4] / 2, e_t[2]]])
+        _, st, _, vmt = ops.fea_integrals(K0, C, B, el, E, (nodes @ Gt.T).ravel())
+        assert np.allclose(st, target[None], atol=1e-12)
+        assert np.allclose(vmt, vm_expect, rtol=1e-12)
+
+
 # ------------------------------------------------------------------ filter
 
 
@@ -360,3 +407,53 @@ def test_sensitivity_central_differences():
         xm[e] -= h
         fd = (comp(xp)[0] - comp(xm)[0]) / (2 * h)
         assert np.isclose(dr[e], fd, rtol=1e-6, atol=1e-9 * abs(c))
+
+
+# ------------------------------------------------------------------ Krylov
+
+def _spd_problem(nel, seed):
+    from oracle import krylov  # noqa: F401
+    el, K0, E, u, cons, n_dof = _rand_problem(nel, seed)
+    A = ops.assemble(K0, el, E, cons, n_dof)
+    b = np.random.default_rng(seed + 1).standard_normal(n_dof)
+    return A, b
+
+
+def test_cg_iterates_equal_scipy_cg():
+    """Unpreconditioned CG is one recurrence: the oracle's iterates are SciPy's (to rounding),
+    and the converged x solves the system."""
+    from oracle import krylov
+    import scipy.sparse.linalg as sla
+    A, b = _spd_problem((6, 4, 3), 3)
+    mine, ref = [], []
+    krylov.cg(lambda v: A @ v, b, tol=1e-10, maxiter=25, callback=lambda x: mine.append(x.copy()))
+    sla.cg(A, b, rtol=1e-10, maxiter=25, callback=lambda x: ref.append(x.copy()))
+    k = min(len(mine), len(ref))
+    assert k >= 20
+    for a, r in zip(mine[:k], ref[:k]):
+        assert np.linalg.norm(a - r) <= 1e-9 * np.linalg.norm(r)
+    x, it = krylov.cg(lambda v: A @ v, b, tol=1e-10, maxiter=5000)
+    assert np.linalg.norm(b - A @ x) <= 1e-10 * np.linalg.norm(b)
+    assert np.allclose(x, spla.spsolve(A.tocsc(), b), rtol=0, atol=1e-7 * np.abs(x).max())
+
+
+def test_gmres_first_cycle_is_minimum_residual():
+    """One GMRES(m) cycle returns the minimiser of |b - A x| over x0 + K_m(A, r0), computed here
+    by dense least squares on an explicit (orthonormalised) Krylov basis; restarts converge."""
+    from oracle import krylov
+    A, b = _spd_problem((5, 3, 3), 5)
+    m = 6
+    x1, it = krylov.gmres(lambda v: A @ v, b, tol=1e-30, restart=m, maxiter=m)
+    assert it == m
+    K = np.empty((b.size, m))
+    v = b.copy()
+    for j in range(m):
+        K[:, j] = v / np.linalg.norm(v)
+        v = A @ K[:, j]
+    Q, _ = np.linalg.qr(K)
+    y = np.linalg.lstsq(A @ Q, b, rcond=None)[0]
+    xr = Q @ y
+    assert np.linalg.norm(x1 - xr) <= 1e-8 * np.linalg.norm(xr)
+    x, it = krylov.gmres(lambda v: A @ v, b, tol=1e-9, restart=20, maxiter=20000)
+    assert np.linalg.norm(b - A @ x) <= 1e-9 * np.linalg.norm(b)
+    assert it % 20 == 0
