@@ -40,7 +40,7 @@ namespace et {
 struct BatchProblem {
   et_batch_problem d{};
   Grid g{};
-  int dof = 0;
+  int dof = 0, k0_flags = 0;
   int64_t n_dof = 0, n_el = 0;
   std::vector<double> K0h, fh, x0h, wh;
   std::vector<uint8_t> consh;
@@ -106,6 +106,7 @@ int setup_problem(BatchProblem& p, int precision) {
   ET_REQUIRE(d.A > 0 && d.volfrac > 0 && d.tol > 0 && d.maxiter >= 1 && d.n_level >= 1,
              "et_batch_create: bad problem parameters");
   p.K0h.assign(d.K0, d.K0 + nd * nd);
+  p.k0_flags = k0_structure(d.dim, p.K0h.data(), kK0Rtol);
   p.consh.assign(d.cons, d.cons + p.n_dof);
   p.fh.assign(d.f, d.f + p.n_dof);
   p.x0h.assign(d.x0, d.x0 + p.n_el);
@@ -157,7 +158,7 @@ int between_solves(BatchProblem& p) {
   const et_batch_problem& d = p.d;
   hipStream_t s = p.stream;
   ET_TRY(dot(p.n_dof, p.f, p.U, p.partial, p.scal, s));
-  ET_TRY(simp_adjoint(p.g, p.K0, p.U, p.xt, d.penal, d.void_, d.n_off, p.off, p.w, p.norm, p.dc, s));
+  ET_TRY(simp_adjoint(p.g, p.K0, p.U, p.xt, d.penal, d.void_, d.n_off, p.off, p.w, p.norm, p.k0_flags, p.dc, s));
   ET_TRY(oc_update(p.n_el, p.x, p.dc, d.A, d.A * p.n_el, d.volfrac, d.move, p.xn, p.ocw, nullptr, s));
   ET_TRY(sqdist(p.n_el, p.xn, p.x, p.partial, p.scal + 1, s));
   copy_kernel<<<grid_for(p.n_el), kBlock, 0, s>>>(p.n_el, p.xn, p.x);

@@ -319,13 +319,13 @@ int et_filter_adj_f64(int dim, const int* nel, int n_off, const int32_t* offsets
 
 int et_simp_adjoint_f64(int dim, const int* nel, const double* K0, const double* U, const double* xt,
                         double penal, double void_, int n_off, const int32_t* offsets, const double* w,
-                        const double* norm, double* out, hipStream_t stream) {
+                        const double* norm, int k0_flags, double* out, hipStream_t stream) {
   ET_GUARD({
     Grid g;
     ET_TRY(et::make_grid(dim, nel, &g));
     ET_REQUIRE(K0 && U && xt && offsets && w && norm && out, "et_simp_adjoint_f64: null pointer");
     ET_REQUIRE(out != xt && out != norm, "et_simp_adjoint_f64: out must not alias xt or norm");
-    return et::simp_adjoint(g, K0, U, xt, penal, void_, n_off, offsets, w, norm, out, stream);
+    return et::simp_adjoint(g, K0, U, xt, penal, void_, n_off, offsets, w, norm, k0_flags, out, stream);
   })
 }
 
@@ -460,6 +460,23 @@ int et_gmres_f64(int dim, const int* nel, const double* K0, const double* E, con
     return et::gmres_solve(g, K0, E, cons, fine, b, x, tol, restart, maxiter, work, work_bytes, iters, residual,
                            stream);
   })
+}
+
+int et_multimat_f64(int64_t n, int m, const double* rho, const double* E_mat, double penal, double void_, int rp,
+                    int mode, double* out, hipStream_t stream) {
+  ET_GUARD({
+    ET_REQUIRE(n > 0 && rho && E_mat && out, "et_multimat_f64: bad arguments");
+    ET_REQUIRE(mode >= 0 && mode <= 2, "et_multimat_f64: mode must be 0, 1 or 2");
+    return et::multimat(n, m, rho, E_mat, penal, void_, rp, mode, out, stream);
+  })
+}
+
+int64_t et_oc_multi_work_size(int64_t n, int m) { return et::oc_multi_work_size(n, m); }
+
+int et_oc_update_multi_f64(int64_t n, int m, const double* x, const double* dc, double A, double vol, double move,
+                           const double* mass, const double* rho_mat, double* x_new, void* work, int* steps,
+                           hipStream_t stream) {
+  ET_GUARD({ return et::oc_update_multi(n, m, x, dc, A, vol, move, mass, rho_mat, x_new, work, steps, stream); })
 }
 
 int64_t et_oc_work_size(int64_t n) { return et::oc_work_size(n); }

@@ -40,7 +40,8 @@
 namespace et {
 
 constexpr int kMaxOff = 1024;
-constexpr int kMaxR = 3;     // halo radius per axis held in the stencil kernels' ring
+constexpr int kMaxR = 2;     // halo radius per axis held in the stencil kernels' ring (rmin <= 3)
+constexpr int kMaxTaps = 512;  // taps held in LDS by the stencil kernels (rmin up to about 4.9)
 constexpr int kCX = 16;      // x planes per workgroup
 
 template <int DIM>
@@ -138,7 +139,7 @@ __host__ __device__ constexpr bool fk_kept(int M, int c, int d) {
 // -u_e^T K0 u_e of element (x, y, z) of a 3-D grid: transformed basis or full product
 template <bool WHT>
 __device__ __forceinline__ double elem_energy3(const Grid& g, int ex, int ey, int ez, const double* __restrict__ U,
-                                               const double* __restrict__ K0, const double* sG) {
+                                               const double* K0, const double* sG) {
   const int nx = g.nx(), nz = g.nz();
   // node index of corner-bit beta = bx + 2 by + 4 bz
   const int64_t m0 = ((int64_t)ey * nx + ex) * nz + ez;
@@ -168,12 +169,15 @@ __device__ __forceinline__ double elem_energy3(const Grid& g, int ex, int ey, in
       asm volatile("" ::: "memory");
     }
 #pragma unroll
-    for (int M = 1; M < 8; ++M)
+    for (int M = 1; M < 8; ++M) {
+      // the coefficients of one M at a time (a clobber per M bounds the loads in flight)
+      asm volatile("" ::: "memory");
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int d = 0; d < 3; ++d)
           if (fk_kept(M, c, d)) e = fma(uh[c][M] * sG[(M - 1) * 9 + c * 3 + d], uh[d][M ^ (1 << c) ^ (1 << d)], e);
+    }
   } else {
     // full product in the element's own DOF order
     double u[8][3];  // corner-bit order
@@ -205,7 +209,7 @@ __device__ __forceinline__ double elem_energy3(const Grid& g, int ex, int ey, in
 }
 
 __device__ inline double elem_energy2(const Grid& g, int ex, int ey, const double* __restrict__ U,
-                                      const double* __restrict__ K0) {
+                                      const double* K0) {
   const int nx = g.nx();
   double ue[8];
 #pragma unroll
@@ -225,35 +229,36 @@ __device__ inline double elem_energy2(const Grid& g, int ex, int ey, const doubl
   return -e;
 }
 
+// out of line: pow's many registers stay out of the stencil loops
+__device__ __attribute__((noinline)) double simp_pow(double x, double e) { return pow(x, e); }
+
+// SIMP factor p x^(p-1); for the usual p = 3 (GP false, chosen on the host) x^2 = x * x,
+// which is pow(x, 2) correctly rounded, and pow's registers stay out of the kernel
+template <bool GP>
+__device__ __forceinline__ double simp_factor(double x, double penal) {
+  return GP ? penal * simp_pow(x, penal - 1.0) : penal * (x * x);
+}
+
 // value staged for element e at (xs, gy, gz) of the stencil grid:
-// MODE 1 x_e, 2 v_e / N_e, 3 s_e / N_e (SIMP adjoint)
-// (EN: element energy in the transformed basis, 1, or as the full product, 2)
+// MODE 1 x_e, 2 v_e / N_e, 3 s_e / N_e (SIMP adjoint; EN 1 transformed-basis energy, 2 full)
 template <int MODE, int DIM, int EN>
 __device__ __forceinline__ double stencil_value(const Grid& g, const SGrid& sg, int xs, int gy, int gz,
                                                 const double* __restrict__ x, const AdjArgs& adj,
-                                                const double* sG) {
+                                                const double* sK, const double* sG) {
   const int64_t e = ((int64_t)xs * sg.Y + gy) * sg.Z + gz;
   if (MODE == 1) return x[e];
   if (MODE == 2) return x[e] / adj.norm[e];
   const double xe = fmin(fmax(adj.xt[e], adj.vmin), 1.0);
-  const double en = DIM == 3 ? elem_energy3<EN == 1>(g, xs, gy, gz, adj.U, adj.K0, sG)
-                             : elem_energy2(g, xs, gz, adj.U, adj.K0);
-  return adj.penal * pow(xe, adj.penal - 1.0) * en / adj.norm[e];
-}
-
-// the same, out of line: the wide-stencil path calls it per tap
-template <int MODE, int DIM, int EN>
-__device__ __attribute__((noinline)) double stencil_value_call(const Grid& g, const SGrid& sg, int xs, int gy,
-                                                                int gz, const double* __restrict__ x,
-                                                                const AdjArgs& adj, const double* sG) {
-  return stencil_value<MODE, DIM, EN>(g, sg, xs, gy, gz, x, adj, sG);
+  const double en = DIM == 3 ? elem_energy3<(EN & 3) == 1>(g, xs, gy, gz, adj.U, sK, sG)
+                             : elem_energy2(g, xs, gz, adj.U, sK);
+  return simp_factor<(EN & 4) != 0>(xe, adj.penal) * en / adj.norm[e];
 }
 
 // MODE 1: forward, 2: transpose (values v / N), 3: SIMP adjoint (values s / N).
-// Tile: TY x TZ outputs per plane (3-D 4 x 64, 2-D 1 x 256), one per thread.
-// The 3-D adjoint is launched twice, EN = 1 (transformed-basis energy) and EN = 2
-// (full product): each workgroup checks K0's structure and the launch that does
-// not apply returns after that check, so each variant keeps only its own registers.
+// Tile: TY x TZ outputs per plane (3-D 4 x 64, 2-D 1 x 256), one per thread.  Ring of
+// 8 element planes (x mod 8): 2 kMaxR + 1 in use plus the one being filled.  EN (MODE 3, 3-D): 1 = the
+// element energy in the transformed basis (element matrices with the brick's reflection
+// structure, chosen on the host), 2 = the full product; EN + 4: a penalty other than 3.
 template <int MODE, int DIM, int EN>
 __global__ __launch_bounds__(256) void stencil_kernel(const Grid g, const SGrid sg, int n_off,
                                                       const int32_t* __restrict__ off,
@@ -264,61 +269,49 @@ __global__ __launch_bounds__(256) void stencil_kernel(const Grid g, const SGrid 
   constexpr int TY = DIM == 3 ? 4 : 1, TZ = 256 / TY;
   constexpr int RYM = DIM == 3 ? kMaxR : 0;  // y halo capacity (2-D grids have one y row)
   constexpr int PY = TY + 2 * RYM, PZ = TZ + 2 * kMaxR, PL = PY * PZ;
-  constexpr int RING = 2 * kMaxR + 1;
+  constexpr int RING = 8;
+  constexpr int NK = MODE == 3 ? (DIM == 3 ? 576 : 64) : 1;
   __shared__ double ring[RING * PL];
-  __shared__ int s_off[kMaxOff];  // packed tap (dx, dy, dz), each + 128
-  __shared__ double s_w[kMaxOff];
-  __shared__ double sG[63];
-  __shared__ int s_r[3], s_wht;
+  __shared__ int s_tap[kMaxTaps];  // dx + 8 in bits 16+, in-plane LDS offset dy * PZ + dz + 2^15 below
+  __shared__ double s_w[kMaxTaps];
+  __shared__ double sK[NK];
+  __shared__ double sG[MODE == 3 ? 63 : 1];
+  __shared__ int s_r[3];
+  __shared__ double s_wsum;
   const int tid = threadIdx.x;
   if (tid < 3) s_r[tid] = 0;
-  if (tid == 0) s_wht = 1;
+  if (MODE == 3)
+    for (int t = tid; t < NK; t += 256) sK[t] = adj.K0[t];
   __syncthreads();
   for (int t = tid; t < n_off; t += 256) {
     const int dx = off[t * DIM];
     const int dy = DIM == 3 ? off[t * 3 + 1] : 0;
     const int dz = DIM == 3 ? off[t * 3 + 2] : off[t * 2 + 1];
-    s_off[t] = ((dx + 128) << 16) | ((dy + 128) << 8) | (dz + 128);
+    s_tap[t] = ((dx + 8) << 16) | (dy * PZ + dz + (1 << 15));
     s_w[t] = w[t];
     atomicMax(&s_r[0], abs(dx));
     atomicMax(&s_r[1], abs(dy));
     atomicMax(&s_r[2], abs(dz));
   }
-  if (MODE == 3 && DIM == 3) {
+  if (MODE == 3 && DIM == 3 && (EN & 3) == 1 && tid < 63) {
     // kept couplings of G = H K0 H / 64
-    if (tid < 63) {
-      const int M = 1 + tid / 9, c = (tid / 3) % 3, d = tid % 3;
-      const int Mp = M ^ (1 << c) ^ (1 << d);
-      double s = 0.0;
-#pragma unroll 1
-      for (int bb = 0; bb < 8; ++bb)
-#pragma unroll 2
-        for (int bp = 0; bp < 8; ++bp) {
-          const double k = adj.K0[(3 * fcorner(bb) + c) * 24 + 3 * fcorner(bp) + d];
-          s += ((__builtin_popcount(M & bb) ^ __builtin_popcount(Mp & bp)) & 1) ? -k : k;
-        }
-      sG[tid] = fk_kept(M, c, d) ? s / 64.0 : 0.0;
-    }
-    __syncthreads();
-    // they must rebuild K0: K0[(b,c),(b',d)] = sum_M H[b][M] G[M][c][d] H[b'][M^e_c^e_d]
-    double kmax = 0.0;
-#pragma unroll 4
-    for (int t = 0; t < 576; ++t) kmax = fmax(kmax, fabs(adj.K0[t]));
-#pragma unroll 1
-    for (int t = tid; t < 576; t += 256) {
-      const int b = t / 72, c = (t / 24) % 3, b2 = (t / 3) % 8, d = t % 3;
-      double s = 0.0;
-      for (int M = 1; M < 8; ++M) {
-        const int Mp = M ^ (1 << c) ^ (1 << d);
-        const double h = ((__builtin_popcount(M & b) ^ __builtin_popcount(Mp & b2)) & 1) ? -1.0 : 1.0;
-        s += h * sG[(M - 1) * 9 + c * 3 + d];
+    const int M = 1 + tid / 9, c = (tid / 3) % 3, d = tid % 3;
+    const int Mp = M ^ (1 << c) ^ (1 << d);
+    double sum = 0.0;
+    for (int bb = 0; bb < 8; ++bb)
+      for (int bp = 0; bp < 8; ++bp) {
+        const double k = sK[(3 * fcorner(bb) + c) * 24 + 3 * fcorner(bp) + d];
+        sum += ((__builtin_popcount(M & bb) ^ __builtin_popcount(Mp & bp)) & 1) ? -k : k;
       }
-      const double k = adj.K0[(3 * fcorner(b) + c) * 24 + 3 * fcorner(b2) + d];
-      if (!(fabs(s - k) <= 1e-12 * kmax)) s_wht = 0;
-    }
+    sG[tid] = fk_kept(M, c, d) ? sum / 64.0 : 0.0;
+  }
+  if (MODE == 1 && tid == 0) {
+    // weight sum of an element whose taps are all in the grid, in the reference's order
+    double ws = 0.0;
+    for (int t = 0; t < n_off; ++t) ws += s_w[t];
+    s_wsum = ws;
   }
   __syncthreads();
-  if (MODE == 3 && DIM == 3 && (EN == 1) != (s_wht != 0)) return;
   const int rx = s_r[0], ry = DIM == 3 ? s_r[1] : 0, rz = s_r[2];
   const bool wide = rx > kMaxR || ry > RYM || rz > kMaxR;
 
@@ -328,15 +321,20 @@ __global__ __launch_bounds__(256) void stencil_kernel(const Grid g, const SGrid 
   const int oy = y0 + ty, oz = z0 + tz;
   const bool mine = oy < sg.Y && oz < sg.Z;
   if (wide) {
-    // taps beyond the ring: direct gathers (adjoint values recomputed per tap)
+    // taps beyond the ring: gathers from memory (the adjoint recomputes each tap's value)
     if (!mine) return;
+#pragma unroll 1
     for (int xo = x0; xo < x1; ++xo) {
       double sv = 0.0, sw = 0.0;
+#pragma unroll 1
       for (int t = 0; t < n_off; ++t) {
-        const int pk = s_off[t];
-        const int a = xo + (pk >> 16) - 128, b = oy + ((pk >> 8) & 0xff) - 128, c = oz + (pk & 0xff) - 128;
+        const int dx = off[t * DIM];
+        const int dy = DIM == 3 ? off[t * 3 + 1] : 0;
+        const int dz = DIM == 3 ? off[t * 3 + 2] : off[t * 2 + 1];
+        const int a = xo + dx, b = oy + dy, c = oz + dz;
         if (a < 0 || a >= sg.X || b < 0 || b >= sg.Y || c < 0 || c >= sg.Z) continue;
-        sv = fma(s_w[t], stencil_value_call<MODE, DIM, EN>(g, sg, a, b, c, x, adj, sG), sv);
+        if (MODE == 3) asm volatile("" ::: "memory");
+        sv = fma(s_w[t], stencil_value<MODE, DIM, EN>(g, sg, a, b, c, x, adj, sK, sG), sv);
         sw += s_w[t];
       }
       const int64_t e = ((int64_t)xo * sg.Y + oy) * sg.Z + oz;
@@ -345,42 +343,77 @@ __global__ __launch_bounds__(256) void stencil_kernel(const Grid g, const SGrid 
     return;
   }
   const int py = TY + 2 * ry, pz = TZ + 2 * rz, pn = py * pz;
-  // stage element plane xs (zero outside the grid) into ring slot xs mod RING
-  auto stage = [&](int xs) {
-    double* pl = ring + (((xs % RING) + RING) % RING) * PL;
-#pragma unroll 1
-    for (int t = tid; t < pn; t += 256) {
-      const int ly = t / pz, lz = t % pz;
-      const int gy = y0 - ry + ly, gz = z0 - rz + lz;
+  // A plane is staged in two halves: fetch() forms this thread's values of plane xs in
+  // registers (zero outside the grid), put() stores them to ring slot xs mod 8.  The loop
+  // fetches plane xo + rx + 1 before it computes plane xo, so the loads are in flight during
+  // the taps; the ring's spare slot (8 > 2 rx + 1) means the store needs no extra barrier.
+  constexpr int KP = (PL + 255) / 256;
+  double pre[KP];
+  auto fetch = [&](int xs) {
+#pragma unroll
+    for (int q = 0; q < KP; ++q) {
+      const int t = tid + 256 * q;
       double v = 0.0;
-      // the G coefficients are re-read from LDS per element rather than held in
-      // registers across the loop (63 doubles would take half the register file)
-      if (MODE == 3) asm volatile("" ::: "memory");
-      if (xs >= 0 && xs < sg.X && gy >= 0 && gy < sg.Y && gz >= 0 && gz < sg.Z)
-        v = stencil_value<MODE, DIM, EN>(g, sg, xs, gy, gz, x, adj, sG);
-      pl[(ly + RYM - ry) * PZ + lz + kMaxR - rz] = v;
+      if (t < pn) {
+        const int ly = t / pz, lz = t - (t / pz) * pz;
+        const int gy = y0 - ry + ly, gz = z0 - rz + lz;
+        // the element matrix / G coefficients are re-read from LDS per element instead of
+        // being held in registers across the loop (they would take most of the register file)
+        if (MODE == 3) asm volatile("" ::: "memory");
+        if (xs >= 0 && xs < sg.X && gy >= 0 && gy < sg.Y && gz >= 0 && gz < sg.Z)
+          v = stencil_value<MODE, DIM, EN>(g, sg, xs, gy, gz, x, adj, sK, sG);
+      }
+      pre[q] = v;
     }
   };
-  for (int xs = x0 - rx; xs < x0 + rx; ++xs) stage(xs);
-  for (int xo = x0; xo < x1; ++xo) {
-    stage(xo + rx);
-    __syncthreads();
-    if (mine) {
-      double sv = 0.0, sw = 0.0;
-      for (int t = 0; t < n_off; ++t) {
-        const int pk = s_off[t];
-        const int dx = (pk >> 16) - 128, dy = ((pk >> 8) & 0xff) - 128, dz = (pk & 0xff) - 128;
-        const int slot = (((xo + dx) % RING) + RING) % RING;
-        sv = fma(s_w[t], ring[slot * PL + (ty + RYM + dy) * PZ + tz + kMaxR + dz], sv);
-        if (MODE == 1) {
-          const int a = xo + dx, b = oy + dy, c = oz + dz;
-          sw += (a >= 0 && a < sg.X && b >= 0 && b < sg.Y && c >= 0 && c < sg.Z) ? s_w[t] : 0.0;
-        }
+  auto put = [&](int xs) {
+    double* pl = ring + (xs & (RING - 1)) * PL;
+#pragma unroll
+    for (int q = 0; q < KP; ++q) {
+      const int t = tid + 256 * q;
+      if (t < pn) {
+        const int ly = t / pz, lz = t - (t / pz) * pz;
+        pl[(ly + RYM - ry) * PZ + lz + kMaxR - rz] = pre[q];
       }
-      const int64_t e = ((int64_t)xo * sg.Y + oy) * sg.Z + oz;
-      y[e] = MODE == 1 ? (sw > 0.0 ? sv / sw : 0.0) : sv;
     }
-    __syncthreads();  // every read of this step precedes the next restaging
+  };
+  for (int xs = x0 - rx; xs < x0 + rx; ++xs) {
+    fetch(xs);
+    put(xs);
+  }
+  fetch(x0 + rx);
+  // interior outputs (every tap in the grid) take the precomputed weight sum
+  const bool yz_in = oy >= ry && oy < sg.Y - ry && oz >= rz && oz < sg.Z - rz;
+  const int me = (ty + RYM) * PZ + tz + kMaxR - (1 << 15);
+  for (int xo = x0; xo < x1; ++xo) {
+    put(xo + rx);
+    __syncthreads();
+    if (xo + 1 < x1) fetch(xo + rx + 1);
+    if (mine) {
+      double sv = 0.0;
+      for (int t = 0; t < n_off; ++t) {
+        const int tp = s_tap[t];
+        const int slot = (xo + (tp >> 16) - 8) & (RING - 1);
+        sv = fma(s_w[t], ring[slot * PL + (tp & 0xffff) + me], sv);
+      }
+      double out = sv;
+      if (MODE == 1) {
+        double sw;
+        if (yz_in && xo >= rx && xo < sg.X - rx) {
+          sw = s_wsum;
+        } else {
+          sw = 0.0;
+          for (int t = 0; t < n_off; ++t) {
+            const int a = xo + off[t * DIM];
+            const int b = oy + (DIM == 3 ? off[t * 3 + 1] : 0);
+            const int c = oz + (DIM == 3 ? off[t * 3 + 2] : off[t * 2 + 1]);
+            sw += (a >= 0 && a < sg.X && b >= 0 && b < sg.Y && c >= 0 && c < sg.Z) ? s_w[t] : 0.0;
+          }
+        }
+        out = sw > 0.0 ? sv / sw : 0.0;
+      }
+      y[((int64_t)xo * sg.Y + oy) * sg.Z + oz] = out;
+    }
   }
 }
 
@@ -421,23 +454,30 @@ int filter_normalizer(const Grid& g, int n_off, const int32_t* off, const double
 int filter_fwd(const Grid& g, int n_off, const int32_t* off, const double* w, const double* x,
                double* y, hipStream_t s) {
   ET_REQUIRE(n_off > 0 && n_off <= kMaxOff, "filter offset count out of range (1..1024)");
+  if (n_off > kMaxTaps) return launch_gather<1>(g, n_off, off, w, nullptr, x, y, s);
   return launch_stencil<1, 0>(g, n_off, off, w, x, AdjArgs{}, y, s);
 }
 
 int filter_adj(const Grid& g, int n_off, const int32_t* off, const double* w, const double* norm,
                const double* v, double* y, hipStream_t s) {
   ET_REQUIRE(n_off > 0 && n_off <= kMaxOff, "filter offset count out of range (1..1024)");
+  if (n_off > kMaxTaps) return launch_gather<2>(g, n_off, off, w, norm, v, y, s);
   AdjArgs a{};
   a.norm = norm;
   return launch_stencil<2, 0>(g, n_off, off, w, v, a, y, s);
 }
 
 int simp_adjoint(const Grid& g, const double* K0, const double* U, const double* xt, double penal, double vmin,
-                 int n_off, const int32_t* off, const double* w, const double* norm, double* y, hipStream_t s) {
-  ET_REQUIRE(n_off > 0 && n_off <= kMaxOff, "filter offset count out of range (1..1024)");
+                 int n_off, const int32_t* off, const double* w, const double* norm, int k0_flags, double* y,
+                 hipStream_t s) {
+  ET_REQUIRE(n_off > 0 && n_off <= kMaxTaps, "simp adjoint: filter offset count out of range (1..512)");
   const AdjArgs a{K0, U, xt, penal, vmin, norm};
-  if (g.dim == 3) ET_TRY((launch_stencil<3, 1>(g, n_off, off, w, nullptr, a, y, s)));
-  return launch_stencil<3, 2>(g, n_off, off, w, nullptr, a, y, s);
+  const bool wht = g.dim == 3 && (k0_flags & K0S_WHT);
+  if (penal == 3.0)
+    return wht ? launch_stencil<3, 1>(g, n_off, off, w, nullptr, a, y, s)
+               : launch_stencil<3, 2>(g, n_off, off, w, nullptr, a, y, s);
+  return wht ? launch_stencil<3, 5>(g, n_off, off, w, nullptr, a, y, s)
+             : launch_stencil<3, 6>(g, n_off, off, w, nullptr, a, y, s);
 }
 
 }  // namespace et

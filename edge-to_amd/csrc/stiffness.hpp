@@ -127,8 +127,10 @@ int filter_adj(const Grid& g, int n_off, const int32_t* off, const double* w, co
                const double* v, double* y, hipStream_t s);
 
 // y = H^T [p clip(xt, void, 1)^(p-1) (-u_e^T K0 u_e)] in one LDS-halo stencil pass
+// (k0_flags: k0_structure of K0; the transformed-basis energy needs K0S_WHT)
 int simp_adjoint(const Grid& g, const double* K0, const double* U, const double* xt, double penal, double vmin,
-                 int n_off, const int32_t* off, const double* w, const double* norm, double* y, hipStream_t s);
+                 int n_off, const int32_t* off, const double* w, const double* norm, int k0_flags, double* y,
+                 hipStream_t s);
 
 // transfers (g = fine grid)
 template <typename T>
@@ -142,6 +144,13 @@ int prolong_op(const Grid& g, int dof, const T* vc, T* vf, hipStream_t s);
 int galerkin(const Grid& g, int dof, const double* Sf, double* Sc, hipStream_t s);
 template <typename T>
 int stencil_apply(const Grid& g, int dof, const T* S, const T* x, T* y, hipStream_t s);
+
+// multi-material SIMP models (multimat.hip): mode 0 E, 1 plain E, 2 dE/drho (n x m)
+int multimat(int64_t n, int m, const double* rho, const double* Emat, double penal, double vmin, int rp, int mode,
+             double* out, hipStream_t s);
+int64_t oc_multi_work_size(int64_t n, int m);
+int oc_update_multi(int64_t n, int m, const double* x, const double* dc, double A, double vol, double move,
+                    const double* mass, const double* rho_mat, double* xn, void* work, int* steps, hipStream_t s);
 
 // unpreconditioned Krylov solvers on the structured operator (krylov.hip)
 int64_t krylov_work_size(int64_t n, int restart);

@@ -5,6 +5,7 @@ Mirrors EDGETO/MaterialModels.py: MaterialModel (:3-28) and SingleMaterial
 the OC multiplier and the volume test are evaluated inside the device OC
 update (et_oc_update_f64) rather than as separate host-synchronising calls.
 """
+import numpy as np
 import torch
 
 from . import _lib
@@ -70,3 +71,83 @@ class SingleMaterial(MaterialModel):
         if self.penalty_schedule is not None:
             return self.penalty_schedule(self.penalty, iteration) == self.penalty
         return True
+
+
+class PenalizedMultiMaterial(MaterialModel):
+    """Discrete multi-material SIMP (MaterialModels.py:100-235): densities rho (n_el, m),
+    E = clip(sum_i E_i r_i^p prod_{j != i} (1 - r_j^p), void, inf), gradient, per-material
+    (or, with rho_mat, total-mass) volume constraints and the OC multiplier.  The elementwise
+    maps run in the library (et_multimat_f64, m <= 4); the constraints are evaluated inside the
+    multi-multiplier OC update (et_oc_update_multi_f64)."""
+
+    _rp = 0
+
+    def __init__(self, E=1.0, nu=0.33, n_material=2, mass=np.array([[0.25], [0.25]]),
+                 E_mat=np.array([[1.0], [0.5]]), rho_mat=None, void=1e-6, penalty=3.0, penalty_schedule=None):
+        super().__init__(E=E, nu=nu, n_sets=n_material)
+        self.n_material = n_material
+        self.mass = np.asarray(mass, dtype=np.float64)
+        self.E_mat = np.asarray(E_mat, dtype=np.float64)
+        self.rho_mat = None if rho_mat is None else np.asarray(rho_mat, dtype=np.float64)
+        self.void = void
+        self.penalty = penalty
+        self.penalty_schedule = penalty_schedule
+        if self.E_mat.shape[0] != self.n_material:
+            raise ValueError("E_mat must have the same number of materials as n_material")
+        if self.rho_mat is not None:
+            if self.rho_mat.shape[0] != self.n_material:
+                raise ValueError("rho_mat must have the same number of materials as n_material")
+            if self.mass.shape[0] != 1:
+                raise ValueError("if rho_mat is not None, mass must have only one value")
+            self.mass = self.mass.reshape(-1)[:1]
+        elif self.mass.shape[0] != self.n_material:
+            raise ValueError("Mass and E_mat must have the same number of materials")
+        self._E_dev = None
+
+    def _pen(self, iteration):
+        if self.penalty_schedule is not None:
+            return self.penalty_schedule(self.penalty, iteration)
+        return self.penalty
+
+    def _emat(self, device):
+        if self._E_dev is None or self._E_dev.device != device:
+            self._E_dev = torch.tensor(self.E_mat.reshape(-1), dtype=torch.float64, device=device)
+        return self._E_dev
+
+    def _call(self, rho, iteration, mode):
+        rho = rho.reshape(-1, self.n_material).contiguous()
+        n = rho.shape[0]
+        out = torch.empty((n, self.n_material) if mode == 2 else (n,), dtype=torch.float64, device=rho.device)
+        _lib.call("et_multimat_f64", n, self.n_material, _lib.f64(rho), _lib.f64(self._emat(rho.device)),
+                  float(self._pen(iteration)), float(self.void), self._rp, mode, _lib.f64(out), _lib.stream())
+        return out
+
+    def __forward__(self, rho, iteration, plain=False, **_):
+        return self._call(rho, iteration, 1 if plain else 0)
+
+    def __backward__(self, rho, iteration, **_):
+        return self._call(rho, iteration, 2)
+
+    def __con__(self, rho, dg, vol, **_):
+        """Constraint flags per material (host booleans; evaluated on the device)."""
+        rho = rho.reshape(-1, self.n_material)
+        A = float(dg.reshape(-1)[0]) if isinstance(dg, torch.Tensor) else float(np.asarray(dg).reshape(-1)[0])
+        if self.rho_mat is None:
+            V = (rho.sum(dim=0) * A).cpu().numpy()
+            return (V / vol - self.mass.reshape(-1)) <= 0
+        V = float((rho.sum(dim=0).cpu().numpy() * A) @ self.rho_mat.reshape(-1))
+        return np.ones(self.n_material, dtype=bool) * (V / vol <= self.mass.reshape(-1)[0])
+
+    def __initvars__(self, nel, dtype=torch.float64, **_):
+        return torch.ones((nel, self.n_material), dtype=torch.float64, device=torch.device("cuda"))
+
+    def is_terminal(self, iteration):
+        if self.penalty_schedule is not None:
+            return self.penalty_schedule(self.penalty, iteration) == self.penalty
+        return True
+
+
+class PenalizedMultiMaterialRP(PenalizedMultiMaterial):
+    """The RAMP-like variant (MaterialModels.py:237-369): w_j = (1 - r_j)^p."""
+
+    _rp = 1

@@ -8,6 +8,7 @@ purpose: rho_init works (the reference reads rho before assignment at
 :321-322; the intended clip(x0, void, 1) is used), and the OC bisection runs on
 the device with one host synchronisation per chunk of steps.
 """
+import ctypes
 import logging
 import time
 
@@ -138,7 +139,31 @@ class TopOpt:
         return strain, stress, energy, vm
 
     # --------------------------------------------------------------- OC / loop
+    def _update_design_multi(self, rho, df):
+        """One multiplier per material (n_sets = m), TopOpt/_cuda.py:242-280."""
+        mm = self.material_model
+        m = mm.n_sets
+        rho = rho.reshape(-1, m).contiguous()
+        df = df.reshape(-1, m).contiguous()
+        n = rho.shape[0]
+        if getattr(self, "_oc_multi_work", None) is None:
+            size = int(_lib.lib().et_oc_multi_work_size(n, m))
+            self._oc_multi_work = torch.empty(size // 8 + 1, dtype=torch.float64, device=rho.device)
+        out = torch.empty_like(rho)
+        mass = np.ascontiguousarray(np.asarray(mm.mass, dtype=np.float64).reshape(-1))
+        rhom = None if mm.rho_mat is None else np.ascontiguousarray(np.asarray(mm.rho_mat, dtype=np.float64).reshape(-1))
+        steps = ctypes.c_int(0)
+        dp = ctypes.POINTER(ctypes.c_double)
+        _lib.call("et_oc_update_multi_f64", n, m, _lib.f64(rho), _lib.f64(df, n * m), float(self.mesh.A_host),
+                  float(self.mesh.volume), float(self.move), mass.ctypes.data_as(dp),
+                  None if rhom is None else rhom.ctypes.data_as(dp), _lib.f64(out, n * m),
+                  _lib.f64(self._oc_multi_work), ctypes.byref(steps), _lib.stream())
+        self._multi_steps = steps.value
+        return out
+
     def update_design(self, rho, df):
+        if self.material_model.base_properties()["n_sets"] > 1:
+            return self._update_design_multi(rho, df)
         rho = rho.reshape(-1).contiguous()
         df = df.reshape(-1).contiguous()
         out = torch.empty_like(rho)
@@ -152,6 +177,8 @@ class TopOpt:
     @property
     def last_oc_steps(self):
         """Bisection steps of the last update_design (reads the device state: synchronises)."""
+        if self.material_model.base_properties()["n_sets"] > 1:
+            return getattr(self, "_multi_steps", 0)
         return int(self._oc_work[:5].view(torch.int32)[9].item())
 
     def _dot(self, a, b):
@@ -164,8 +191,26 @@ class TopOpt:
                   _lib.stream())
         return float(np.sqrt(self._scalar[1].item() / a.numel()))
 
+    def _solve_and_process_multi(self, rho, iteration):
+        """m materials: each column filtered, E from the multi-material map, dc_k = H^T
+        (dE/dr_k * (-u_e^T K0 u_e)) per column (TopOpt/_cuda.py:282-302)."""
+        mm = self.material_model
+        m = mm.n_sets
+        rho = rho.reshape(-1, m)
+        rho_ = torch.stack([self.filter_kernel.dot(rho[:, k].contiguous()) for k in range(m)], dim=1).contiguous()
+        E = mm(rho_, iteration)
+        self.kernel.set_rho(E)
+        U, residual = self.solver.solve(self.f, use_last=self.reuse_sol)
+        compliance = self._dot(self.f, U)
+        dk = self.kernel.process_grad(U)
+        dr = mm.grad(rho_, iteration) * dk.reshape(-1, 1)
+        dr = torch.stack([self.filter_kernel._rmatvec(dr[:, k].contiguous()) for k in range(m)], dim=1)
+        return compliance, dr, residual
+
     def solve_and_process(self, rho, iteration):
         mm = self.material_model
+        if mm.base_properties()["n_sets"] > 1:
+            return self._solve_and_process_multi(rho, iteration)
         rho_ = self.filter_kernel.dot(rho)
         rho__ = mm(rho_, iteration)
         self.kernel.set_rho(rho__)
@@ -182,12 +227,21 @@ class TopOpt:
     def optimize(self, save_comp_history=False, save_change_history=False, save_rho_history=False,
                  rho_init=None):
         flag = False
+        multi = self.material_model.base_properties()["n_sets"] > 1
         if rho_init is None:
             rho = self.material_model.init_desvars(self.kernel.elements.shape[0])
+        elif multi:
+            # clip(x0, void, 1) per material (the intent of :321-322)
+            x0 = torch.as_tensor(rho_init, dtype=torch.float64, device=self.f.device).reshape(-1).contiguous()
+            rho = torch.empty_like(x0)
+            _lib.call("et_simp_forward_f64", x0.numel(), _lib.f64(x0), 1.0, float(self.material_model.void), 1,
+                      _lib.f64(rho, x0.numel()), _lib.stream())
+            rho = rho.reshape(-1, self.material_model.n_sets)
         else:
             x0 = torch.as_tensor(rho_init, dtype=torch.float64, device=self.f.device).reshape(-1)
             rho = self.material_model(x0.contiguous(), 0, plain=True)
-        self.kernel.set_rho(rho)
+        if not multi:
+            self.kernel.set_rho(rho)
         t0 = time.perf_counter()
         comp, df, res = self.solve_and_process(rho, 0)
         self.iteration_log = [dict(it=-1, comp=comp, res=res, pcg=self.solver.last_iterations,

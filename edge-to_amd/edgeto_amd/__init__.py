@@ -20,7 +20,8 @@ from .geom import (CuStructuredFilter2D, CuStructuredFilter3D, CuStructuredMesh2
                    CuStructuredMesh3D, FilterKernel, StructuredFilter2D, StructuredFilter3D,
                    StructuredMesh2D, StructuredMesh3D)
 from .kernels import CuStructuredStiffnessKernel, StiffnessKernel, StructuredStiffnessKernel
-from .MaterialModels import MaterialModel, SingleMaterial
+from .MaterialModels import (MaterialModel, PenalizedMultiMaterial, PenalizedMultiMaterialRP,
+                             SingleMaterial)
 from .physics import (auto_stiffness, hexahedron_element_stiffness,
                       quadrilateral_element_stiffness)
 from .solvers import CG, GMRES, CuCG, CuGMRES, CuMultiGrid, MultiGrid, Solver
@@ -37,4 +38,4 @@ __all__ = ["CuStructuredMesh2D", "CuStructuredMesh3D", "CuStructuredFilter2D",
            "TopOpt", "FilterKernel", "StiffnessKernel", "MaterialModel", "Solver",
            "quadrilateral_element_stiffness", "hexahedron_element_stiffness", "compliance",
            "ComplianceFunction", "energy_compliance", "CuCG", "CuGMRES", "CG", "GMRES", "BatchTopOpt",
-           "gmg_levels", "random_problem"]
+           "gmg_levels", "random_problem", "PenalizedMultiMaterial", "PenalizedMultiMaterialRP"]

@@ -61,7 +61,7 @@ _SIGS = {
     "et_filter_normalizer_f64": [_I, _IP, _I, _P, _P, _P, _P],
     "et_filter_fwd_f64": [_I, _IP, _I, _P, _P, _P, _P, _P],
     "et_filter_adj_f64": [_I, _IP, _I, _P, _P, _P, _P, _P, _P],
-    "et_simp_adjoint_f64": [_I, _IP, _P, _P, _P, _D, _D, _I, _P, _P, _P, _P, _P],
+    "et_simp_adjoint_f64": [_I, _IP, _P, _P, _P, _D, _D, _I, _P, _P, _P, _I, _P, _P],
     "et_restrict_f64": [_I, _IP, _I, _P, _P, _P],
     "et_prolong_f64": [_I, _IP, _I, _P, _P, _P],
     "et_stencil_from_operator_f64": [_I, _IP, _P, _P, _P, _P, _P],
@@ -71,6 +71,10 @@ _SIGS = {
     "et_dot_f64": [_I64, _P, _P, _P, _P],
     "et_sqdist_f64": [_I64, _P, _P, _P, _P],
     "et_oc_work_size": [_I64],
+    "et_multimat_f64": [_I64, _I, _P, _P, _D, _D, _I, _I, _P, _P],
+    "et_oc_multi_work_size": [_I64, _I],
+    "et_oc_update_multi_f64": [_I64, _I, _P, _P, _D, _D, _D, ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_double), _P, _P, ctypes.POINTER(ctypes.c_int), _P],
     "et_krylov_work_size": [_I64, _I],
     "et_cg_f64": [_I, _IP, _P, _P, _P, _P, _P, _D, _I, _P, _I64, ctypes.POINTER(ctypes.c_int),
                   ctypes.POINTER(ctypes.c_double), _P],
@@ -89,7 +93,7 @@ _SIGS = {
     "et_mg_profile_read": [_P, _P, _P],
 }
 
-_I64_RET = ("et_oc_work_size", "et_krylov_work_size")
+_I64_RET = ("et_oc_work_size", "et_krylov_work_size", "et_oc_multi_work_size")
 
 PROF_KINDS = ["l0_jacobi", "l0_residual", "l0_jacobi_add", "l0_scale", "matvec64", "coarse_ops",
               "transfer", "coarse_solve", "pcg_vector", "setup", "l0_jacobi0"]

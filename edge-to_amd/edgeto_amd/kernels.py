@@ -48,6 +48,18 @@ class StructuredStiffnessKernel(StiffnessKernel):
         self.has_con = False
         self._star = None
 
+    @property
+    def k0_flags(self):
+        """et_k0_structure of the element matrix (host check, computed once)."""
+        if getattr(self, "_k0_flags", None) is None:
+            import ctypes
+            K = np.ascontiguousarray(self.mesh.K_single_host, dtype=np.float64)
+            f = ctypes.c_int(0)
+            _lib.call("et_k0_structure", self._dim, K.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                      ctypes.byref(f))
+            self._k0_flags = f.value
+        return self._k0_flags
+
     # --- reference index arrays (kernels/_cuda.py:77-96), built on first use ---
     def _build_star(self):
         flat = self.elements_flat.cpu().numpy()

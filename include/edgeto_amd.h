@@ -217,10 +217,13 @@ int et_filter_adj_f64(int dim, const int* nel, int n_off, const int32_t* offsets
  * i.e. et_sensitivity_f64 followed by et_filter_adj_f64 (CuTopOpt.solve_and_process,
  * TopOpt/_cuda.py:293-300) with s_e / N_e formed once per element while it is staged
  * in LDS.  U: n_dof displacement (constrained DOFs included); xt: filtered design;
- * offsets, w, norm: the filter's table and normaliser. */
+ * offsets, w, norm: the filter's table and normaliser.  k0_flags: et_k0_structure of
+ * the same K0 (host check, once per matrix); with ET_K0_WHT set the element energy is
+ * evaluated in the corner Walsh-Hadamard basis (~140 flops instead of 1152), otherwise as
+ * the full product.  Asynchronous (no host round trip). */
 int et_simp_adjoint_f64(int dim, const int* nel, const double* K0, const double* U, const double* xt,
                         double penal, double void_, int n_off, const int32_t* offsets, const double* w,
-                        const double* norm, double* out, hipStream_t stream);
+                        const double* norm, int k0_flags, double* out, hipStream_t stream);
 
 /* ------------------------------------------------- multigrid transfers --- */
 /* Restriction fine -> coarse (nel = fine element counts): restriction_{2d,3d}_kernel
@@ -261,6 +264,23 @@ int et_dot_f64(int64_t n, const double* a, const double* b, double* out, hipStre
 /* *out = sum (a - b)^2 (device scalar); the RMS design change of the SIMP loop
  * (TopOpt/_cuda.py:350) is sqrt(*out / n). */
 int et_sqdist_f64(int64_t n, const double* a, const double* b, double* out, hipStream_t stream);
+
+/* ------------------------------------------------------ multi-material --- */
+/* PenalizedMultiMaterial (rp = 0) / PenalizedMultiMaterialRP (rp = 1), MaterialModels.py:100-369,
+ * for m <= 4 materials: rho n x m (row-major, device), E_mat m moduli (device).
+ * mode 0: out[n] = E (__forward__); 1: out[n] = clip(rho . E_mat, void, 1) (plain);
+ * 2: out[n x m] = dE/drho (__backward__). */
+int et_multimat_f64(int64_t n, int m, const double* rho, const double* E_mat, double penal, double void_, int rp,
+                    int mode, double* out, hipStream_t stream);
+/* OC update with one multiplier per material, bisected in lockstep until every bracket closes
+ * (TopOpt/_cuda.py:242-280 with n_sets = m; constraint PenalizedMultiMaterial.__con__).
+ * x, dc, x_new: n x m device arrays; mass (host): m fractions, or 1 when rho_mat (host, m
+ * material densities) is given (then one mass constraint for all materials).  work:
+ * et_oc_multi_work_size(n, m) bytes, 8-byte aligned.  Synchronises `stream`. */
+int64_t et_oc_multi_work_size(int64_t n, int m);
+int et_oc_update_multi_f64(int64_t n, int m, const double* x, const double* dc, double A, double vol, double move,
+                           const double* mass, const double* rho_mat, double* x_new, void* work, int* steps,
+                           hipStream_t stream);
 
 /* ------------------------------------------------------ Krylov solvers --- */
 /* Unpreconditioned solvers of A_hat x = b on the matrix-free operator (the

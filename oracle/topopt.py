@@ -134,3 +134,101 @@ def optimize(prob, material, solver, x0, max_iter, move=0.2, ch_tol=1e-4, fun_to
             flag = True
             break
     return x, flag, hist
+
+
+class PenalizedMultiMaterial:
+    """Discrete material optimisation with m candidate materials (MaterialModels.py:100-235;
+    rp=True: PenalizedMultiMaterialRP, :237-369).  Densities rho (n_el, m), per-material moduli
+    E_mat (m,):
+      forward   E = clip(sum_i E_i r_i^p prod_{j != i} (1 - r_j^p), void, inf)
+                rp: prod_{j != i} (1 - r_j)^p                          (:145-176)
+      plain     clip(rho @ E_mat, void, 1)                              (:146-149)
+      grad      dE/dr_k = E_k p r_k^(p-1) prod_{j != k} w_j
+                          - sum_{i != k} E_i r_i^p dw_k prod_{j != i, k} w_j,
+                w_j = 1 - r_j^p (rp: (1 - r_j)^p), dw_k = p r_k^(p-1) (rp: p (1 - r_k)^(p-1))
+                (the unclipped expression, as :178-206 evaluates it; r = clip(rho, void, 1))
+      constraint per material sum_e r_ek A / vol <= mass_k, or with rho_mat the total
+                sum_e sum_k r_ek A rho_mat_k / vol <= mass for every material  (:208-221)
+      ocP       rho * nan_to_num(sqrt(-df / A)), 1e-3 everywhere if all zero (:223-229)."""
+
+    def __init__(self, E_mat, mass, penalty=3.0, void=1e-6, rho_mat=None, rp=False):
+        self.E_mat = np.asarray(E_mat, dtype=np.float64).reshape(-1)
+        self.m = self.E_mat.size
+        self.mass = np.asarray(mass, dtype=np.float64).reshape(-1)
+        self.rho_mat = None if rho_mat is None else np.asarray(rho_mat, dtype=np.float64).reshape(-1)
+        self.penalty, self.void, self.rp = penalty, void, rp
+
+    def _w(self, r):
+        p = self.penalty
+        return (1.0 - r) ** p if self.rp else 1.0 - r ** p
+
+    def forward(self, rho, plain=False):
+        if plain:
+            return np.clip(rho @ self.E_mat, self.void, 1.0)
+        p = self.penalty
+        r = np.clip(rho, self.void, 1.0)
+        w = self._w(r)
+        E = np.zeros(r.shape[0])
+        for i in range(self.m):
+            t = r[:, i] ** p
+            for j in range(self.m):
+                if j != i:
+                    t = t * w[:, j]
+            E += self.E_mat[i] * t
+        return np.clip(E, self.void, np.inf)
+
+    def grad(self, rho):
+        p = self.penalty
+        r = np.clip(rho, self.void, 1.0)
+        w = self._w(r)
+        dw = p * (1.0 - r) ** (p - 1) if self.rp else p * r ** (p - 1)
+        g = np.zeros_like(r)
+        for k in range(self.m):
+            t = self.E_mat[k] * p * r[:, k] ** (p - 1)
+            for j in range(self.m):
+                if j != k:
+                    t = t * w[:, j]
+            for i in range(self.m):
+                if i == k:
+                    continue
+                s = self.E_mat[i] * r[:, i] ** p * dw[:, k]
+                for j in range(self.m):
+                    if j != i and j != k:
+                        s = s * w[:, j]
+                t = t - s
+            g[:, k] = t
+        return g
+
+    def constraint(self, x, A, vol):
+        if self.rho_mat is None:
+            return (x * A).sum(axis=0) / vol - self.mass <= 0
+        return np.full(self.m, ((x * A) @ self.rho_mat).sum() / vol <= self.mass[0])
+
+    def ocP(self, dc, A, x):
+        with np.errstate(invalid="ignore", divide="ignore"):
+            o = x * np.nan_to_num(np.sqrt(-dc / A), nan=0)
+        if np.abs(o).sum() == 0:
+            o = np.ones_like(o) * 1e-3
+        return o
+
+
+def oc_update_multi(x, dc, A, vol, move, material):
+    """OC with one multiplier per material, bisected in lockstep until every bracket has closed
+    (TopOpt/_cuda.py:242-280 with n_sets = m: lmid has one entry per material and the loop runs
+    while any (l2 - l1) / (l2 + l1) > 1e-4).  x, dc: (n_el, m)."""
+    m = x.shape[1]
+    xU = np.clip(x + move, 0.0, 1.0)
+    xL = np.clip(x - move, 0.0, 1.0)
+    ocP = np.maximum(1e-12, material.ocP(dc, A, x))
+    l1, l2 = np.full(m, 1e-9), np.full(m, 1e9)
+    lmid = 0.5 * (l1 + l2)
+    xn = np.maximum(0, np.maximum(xL, np.minimum(1.0, np.minimum(xU, ocP / lmid))))
+    steps = 0
+    while np.any((l2 - l1) / (l2 + l1) > 1e-4):
+        lmid = 0.5 * (l1 + l2)
+        xn = np.maximum(0, np.maximum(xL, np.minimum(1.0, np.minimum(xU, ocP / lmid))))
+        valid = material.constraint(xn, A, vol)
+        l2 = np.where(valid, lmid, l2)
+        l1 = np.where(valid, l1, lmid)
+        steps += 1
+    return xn, steps

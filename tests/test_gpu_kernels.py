@@ -534,9 +534,18 @@ def test_simp_adjoint_fused(nel, rmin, spacing, k0):
     norm = ofilt.normalizer(nel, offs, w)
     n = int(np.prod(nel))
     xt = np.random.default_rng(8).uniform(0.0, 1.0, n)
-    out = torch.empty(n, dtype=torch.float64, device="cuda")
-    L.call("et_simp_adjoint_f64", dim, L.nel_arg(nel), L.f64(dev(K0)), L.f64(dev(p["u"])), L.f64(dev(xt)), 3.0,
-           1e-9, len(w), L.i32(dev(offs.astype(np.int32))), L.f64(dev(w)), L.f64(dev(norm)), L.f64(out), L.stream())
+    flags = ctypes.c_int(0)
+    K0c = np.ascontiguousarray(K0)
+    L.call("et_k0_structure", dim, K0c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(flags))
     mat = otop.SingleMaterial(void=1e-9, penalty=3.0)
     ref = ofilt.transpose(mat.grad(xt) * oops.process_dk(K0, p["el"], p["u"]), nel, offs, w, norm)
-    assert rel(out.cpu().numpy(), ref) < 1e-12
+    # the library's own structure flags, and 0 (the full-product energy for every matrix)
+    for fl in sorted({flags.value, 0}):
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        L.call("et_simp_adjoint_f64", dim, L.nel_arg(nel), L.f64(dev(K0)), L.f64(dev(p["u"])), L.f64(dev(xt)), 3.0,
+               1e-9, len(w), L.i32(dev(offs.astype(np.int32))), L.f64(dev(w)), L.f64(dev(norm)), fl, L.f64(out),
+               L.stream())
+        err = rel(out.cpu().numpy(), ref)
+        assert err < 1e-12, (fl, err)
+    if k0 == "rotated":
+        assert not flags.value & 4  # no transformed-basis structure: the full product ran

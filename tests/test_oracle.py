@@ -457,3 +457,48 @@ def test_gmres_first_cycle_is_minimum_residual():
     x, it = krylov.gmres(lambda v: A @ v, b, tol=1e-9, restart=20, maxiter=20000)
     assert np.linalg.norm(b - A @ x) <= 1e-9 * np.linalg.norm(b)
     assert it % 20 == 0
+
+
+# ---------------------------------------------------------- multi-material
+
+@pytest.mark.parametrize("rp", [False, True])
+def test_multimaterial_model_pins(rp):
+    """The restated PenalizedMultiMaterial(RP): the gradient is the derivative of the forward map
+    (central differences), one material reduces to E_1 r^p, two materials at a vertex give the
+    pure material, plain is the clipped linear mix."""
+    rng = np.random.default_rng(2)
+    mm = topopt.PenalizedMultiMaterial([1.0, 0.5, 0.2], [0.2, 0.1, 0.1], rp=rp)
+    r = rng.uniform(0.05, 0.95, (40, 3))
+    g = mm.grad(r)
+    h = 1e-6
+    for k in range(3):
+        a, b = r.copy(), r.copy()
+        a[:, k] += h
+        b[:, k] -= h
+        fd = (mm.forward(a) - mm.forward(b)) / (2 * h)
+        assert np.max(np.abs(fd - g[:, k])) <= 1e-8 * np.max(np.abs(g))
+    one = topopt.PenalizedMultiMaterial([0.7], [0.3], rp=rp)
+    x = rng.uniform(0.05, 1.0, (30, 1))
+    assert np.allclose(one.forward(x), 0.7 * x[:, 0] ** 3, rtol=1e-15)
+    two = topopt.PenalizedMultiMaterial([1.0, 0.5], [0.2, 0.2], rp=rp, void=1e-9)
+    assert np.isclose(two.forward(np.array([[1.0, 0.0]]))[0], 1.0)
+    assert np.isclose(two.forward(np.array([[0.0, 1.0]]))[0], 0.5)
+    assert np.allclose(two.forward(np.array([[0.3, 0.4]]), plain=True), 0.3 + 0.2)
+
+
+def test_multimaterial_oc_invariants():
+    """Lockstep multi-multiplier OC: bounds, move limits, each material's volume within its
+    mass target to the bisection tolerance, and the total-mass variant."""
+    rng = np.random.default_rng(4)
+    n = 2000
+    x = rng.uniform(0.05, 0.6, (n, 2))
+    dc = -rng.uniform(0.0, 2.0, (n, 2))
+    mm = topopt.PenalizedMultiMaterial([1.0, 0.5], [0.25, 0.15])
+    xn, steps = topopt.oc_update_multi(x, dc, 1.0, float(n), 0.2, mm)
+    assert steps > 10 and np.all(xn >= 0) and np.all(xn <= 1)
+    assert np.all(np.abs(xn - x) <= 0.2 + 1e-15)
+    vol = xn.mean(axis=0)
+    assert np.all(np.abs(vol - np.array([0.25, 0.15])) < 2e-3)
+    mt = topopt.PenalizedMultiMaterial([1.0, 0.5], [[0.3]], rho_mat=[1.0, 2.0])
+    xt, _ = topopt.oc_update_multi(x, dc, 1.0, float(n), 1.0, mt)  # move 1: the target is reachable
+    assert abs((xt.mean(axis=0) @ np.array([1.0, 2.0])) - 0.3) < 2e-3
