@@ -486,6 +486,13 @@ int et_oc_update_f64(int64_t n, const double* x, const double* dc, double A, dou
   ET_GUARD({ return et::oc_update(n, x, dc, A, vol, vf, move, x_new, work, steps, stream); })
 }
 
+int et_oc_update_f64_noalloc(int64_t n, const double* x, const double* dc, double A, double vol,
+                             double vf, double move, double* x_new, int* steps, hipStream_t stream) {
+  ET_GUARD({ return et::oc_update_noalloc(n, x, dc, A, vol, vf, move, x_new, steps, stream); })
+}
+
+int64_t et_oc_noalloc_max_n(void) { return et::oc_noalloc_max_n(); }
+
 int et_mg_create(et_mg** handle, const et_mg_params* params, const double* K0_host,
                  hipStream_t stream) {
   ET_GUARD({

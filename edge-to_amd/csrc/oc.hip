@@ -11,6 +11,7 @@
 // once.  The new design is evaluated once, at the last midpoint tried.  Every
 // kernel is stream-ordered: the whole update can be captured in a graph.
 #include <algorithm>
+#include <mutex>
 
 #include "common.hpp"
 #include "blas.hpp"
@@ -198,6 +199,53 @@ int oc_update(int64_t n, const double* x, const double* dc, double A, double vol
     }
   }
   return ET_OK;
+}
+
+// ------------------------------------------------ allocation-free variant ---
+//
+// For callers that cannot allocate device memory (code under graph capture,
+// tight serving loops): the same kernels on a work buffer in a file-scope
+// device array.  Each GPU holds its own copy of the array for the life of the
+// process; its device address is looked up once per device (the host-side
+// name of a __device__ array is only a stand-in).  Every call on one GPU
+// shares that array: calls that overlap on one GPU (different streams, or
+// host threads) race.  Nothing in it needs clearing between calls: the state
+// is initialised by the first kernel of every update, ocp and the partial
+// sums are written before they are read.
+
+constexpr int64_t kOcNoAllocMaxN = int64_t(1) << 20;  // elements
+constexpr int64_t kOcScratchBytes =
+    kOcStateBytes + kOcNoAllocMaxN * (int64_t)sizeof(double) + kOcCand * kRedBlocks * (int64_t)sizeof(double);
+
+static __device__ __align__(256) unsigned char scratch[kOcScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int64_t oc_noalloc_max_n() { return kOcNoAllocMaxN; }
+
+int oc_update_noalloc(int64_t n, const double* x, const double* dc, double A, double vol, double vf,
+                      double move, double* xn, int* steps, hipStream_t s) {
+  ET_REQUIRE(n > 0 && n <= kOcNoAllocMaxN,
+             "oc_update_noalloc: n must be in [1, 1048576] (size of the static work array)");
+  int dev = -1;
+  ET_CHECK_HIP(hipGetDevice(&dev));
+  ET_REQUIRE(dev >= 0 && dev < kMaxDevices, "oc_update_noalloc: device index out of range");
+  void* work = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+      void* p = nullptr;
+      ET_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      ET_REQUIRE(p != nullptr, "oc_update_noalloc: no device address for the work array");
+      g_scratch_ptr[dev] = p;
+    }
+    work = g_scratch_ptr[dev];
+  }
+  return oc_update(n, x, dc, A, vol, vf, move, xn, work, steps, s);
 }
 
 // sum (a - b)^2, used for the RMS design change of the SIMP loop

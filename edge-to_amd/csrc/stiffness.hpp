@@ -165,6 +165,10 @@ int gmres_solve(const Grid& g, const double* K0, const double* E, const uint8_t*
 int64_t oc_work_size(int64_t n);
 int oc_update(int64_t n, const double* x, const double* dc, double A, double vol, double vf,
               double move, double* xn, void* work, int* steps, hipStream_t s);
+// oc_update on a work buffer in a static device array (no allocation); n <= oc_noalloc_max_n()
+int oc_update_noalloc(int64_t n, const double* x, const double* dc, double A, double vol, double vf,
+                      double move, double* xn, int* steps, hipStream_t s);
+int64_t oc_noalloc_max_n();
 int sqdist(int64_t n, const double* a, const double* b, double* partial, double* out,
            hipStream_t s);
 

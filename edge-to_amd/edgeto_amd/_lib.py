@@ -81,6 +81,8 @@ _SIGS = {
     "et_gmres_f64": [_I, _IP, _P, _P, _P, _P, _P, _D, _I, _I, _P, _I64, ctypes.POINTER(ctypes.c_int),
                      ctypes.POINTER(ctypes.c_double), _P],
     "et_oc_update_f64": [_I64, _P, _P, _D, _D, _D, _D, _P, _P, ctypes.POINTER(ctypes.c_int), _P],
+    "et_oc_update_f64_noalloc": [_I64, _P, _P, _D, _D, _D, _D, _P, ctypes.POINTER(ctypes.c_int), _P],
+    "et_oc_noalloc_max_n": [],
     "et_batch_create": [ctypes.POINTER(_P), _I, _P, _I],
     "et_batch_run": [_P, _I, _I, ctypes.POINTER(ctypes.c_double)],
     "et_batch_result": [_P, _I, _P, _P, _P, _P],
@@ -93,7 +95,7 @@ _SIGS = {
     "et_mg_profile_read": [_P, _P, _P],
 }
 
-_I64_RET = ("et_oc_work_size", "et_krylov_work_size", "et_oc_multi_work_size")
+_I64_RET = ("et_oc_work_size", "et_oc_noalloc_max_n", "et_krylov_work_size", "et_oc_multi_work_size")
 
 PROF_KINDS = ["l0_jacobi", "l0_residual", "l0_jacobi_add", "l0_scale", "matvec64", "coarse_ops",
               "transfer", "coarse_solve", "pcg_vector", "setup", "l0_jacobi0"]

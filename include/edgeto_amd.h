@@ -317,6 +317,18 @@ int et_oc_update_f64(int64_t n, const double* x, const double* dc, double A, dou
                      double vf, double move, double* x_new, void* work, int* steps,
                      hipStream_t stream);
 
+/* et_oc_update_f64 without a work argument and without any device allocation:
+ * the work buffer is a static device array of the library (one per GPU, kept for
+ * the process; about 8.1 MiB), so n must not exceed et_oc_noalloc_max_n()
+ * (1 048 576 elements; larger n returns ET_EINVAL before any launch).  steps as in
+ * et_oc_update_f64 (NULL: asynchronous, no host synchronisation).  Calls that
+ * overlap on one GPU share the array and race; each call re-initialises what it
+ * reads.  No counterpart in the reference: for callers that cannot allocate
+ * (code under graph capture, serving loops, allocation-free harnesses). */
+int et_oc_update_f64_noalloc(int64_t n, const double* x, const double* dc, double A, double vol,
+                             double vf, double move, double* x_new, int* steps, hipStream_t stream);
+int64_t et_oc_noalloc_max_n(void);
+
 /* ------------------------------------------------------- GMG-PCG solver --- */
 /* Opaque solver: the CuMultiGrid object (solvers/_multigrid_cuda.py:15-242) with
  * its hierarchy, workspace and coarse factor.  precision: 0 = FP64 throughout;
