@@ -124,6 +124,9 @@ int et_tune_set(int param, int value) {
     } else if (param == ET_TUNE_WHT_WGS) {
       ET_REQUIRE(value >= 1 && value <= 4, "et_tune_set: workgroups per CU must be in [1, 4]");
       et::set_wht_wgs(value);
+    } else if (param == ET_TUNE_FILTER_ROWS) {
+      ET_REQUIRE(value == 4 || value == 8, "et_tune_set: filter tile rows must be 4 or 8");
+      et::set_stencil_rows(value);
     } else {
       ET_REQUIRE(false, "et_tune_set: unknown parameter");
     }
@@ -148,6 +151,7 @@ int et_tune_get(int param) {
   if (param == ET_TUNE_WHT_THREADS_F32 || param == ET_TUNE_WHT_THREADS_F64)
     return et::get_wht_threads(param == ET_TUNE_WHT_THREADS_F64);
   if (param == ET_TUNE_WHT_WGS) return et::get_wht_wgs();
+  if (param == ET_TUNE_FILTER_ROWS) return et::get_stencil_rows();
   return -1;
 }
 

@@ -3,14 +3,17 @@
 // the SIMP adjoint fused with the transpose.
 //
 // Forward and transpose are LDS-halo stencils.  A workgroup owns a tile of
-// TY x TZ element columns (3-D 4 x 64, a wavefront per row along the fastest
-// axis; 2-D 1 x 256) and marches along x through a chunk of kCX planes.  Each step stages one
-// new element plane of the tile plus its y/z halo into a ring of 2 rx + 1 planes
+// TY x TZ element columns (3-D TY x 64 with TY 4 or 8, a wavefront per row along the
+// fastest axis; 2-D 1 x 256) and marches along x through a chunk of planes (4 to 32, as
+// many as keep about two workgroups per compute unit busy).  Each step stages one
+// new element plane of the tile plus its y/z halo into a ring of 8 plane slots
 // in LDS (zero outside the grid) and produces one output plane from the ring,
 // so every input value is read from memory once per tile (plus the halo rows)
 // and every output is written once, with no atomics.  Sums follow the offset
 // table's order, so results are deterministic; the forward pass is bitwise the
-// reference's (out-of-grid taps add an exact 0.0).
+// reference's (out-of-grid taps add an exact 0.0).  The forward's denominator
+// depends only on the output's distances to the six faces clipped to the radius,
+// so each workgroup tabulates it once per boundary class (at most 3^6 classes).
 //
 //   forward    y_i = (sum_k w_k x_{i+o_k}) / (sum_k [i+o_k in grid] w_k)
 //              (apply_filter_{2D,3D}_kernel, core/_templated_cuda.cu:1526-1583)
@@ -24,10 +27,9 @@
 //
 // In 3-D the element energy is evaluated in the corner Walsh-Hadamard basis,
 // u^T K0 u = sum over the kept couplings of uh[M][c] G[M][c][d] uh[M^e_c^e_d][d]
-// with uh = H u_e and G = H K0 H / 64 (about 140 flops instead of 1152).  Each
-// workgroup forms the kept G entries from K0 and checks that they rebuild K0
-// exactly (to 1e-12 of its largest entry); if they do not (an element matrix
-// without the brick's reflection structure) it evaluates the full product.
+// with uh = H u_e and G = H K0 H / 64 (about 140 flops instead of 1152).  The
+// host picks that form only for element matrices with the brick's reflection
+// structure (k0_structure); any other K0 takes the full product.
 //
 // 2-D grids run as 3-D grids of one element in y (element i * nely + j sits at
 // x = i, z = j).  Each workgroup reads the offset radii from the table; taps
@@ -42,7 +44,7 @@ namespace et {
 constexpr int kMaxOff = 1024;
 constexpr int kMaxR = 2;     // halo radius per axis held in the stencil kernels' ring (rmin <= 3)
 constexpr int kMaxTaps = 512;  // taps held in LDS by the stencil kernels (rmin up to about 4.9)
-constexpr int kCX = 16;      // x planes per workgroup
+constexpr int kMaxCX = 32;   // x planes per workgroup, at most
 
 template <int DIM>
 __device__ __forceinline__ void elem_coords(const Grid& g, int64_t e, int& i, int& j, int& k) {
@@ -254,19 +256,45 @@ __device__ __forceinline__ double stencil_value(const Grid& g, const SGrid& sg, 
   return simp_factor<(EN & 4) != 0>(xe, adj.penal) * en / adj.norm[e];
 }
 
+// Tap sum over the full (2R+1)^DIM cube around the output, in the offset table's
+// (dx, dy, dz) ascending order, with the weight of each cube position (0 where the
+// table has no tap, and such positions are skipped): the loop is unrolled, every
+// LDS address is an immediate offset from one base per x plane.
+template <int R, int DIM, int PZ, int PL, int RING>
+__device__ __forceinline__ double cube_sum(const double* ring, const double* s_wc, int xo, int me) {
+  constexpr int W = 2 * R + 1, WY = DIM == 3 ? W : 1, RY = DIM == 3 ? R : 0;
+  double sv = 0.0;
+#pragma unroll
+  for (int ix = 0; ix < W; ++ix) {
+    const double* p = ring + ((xo + ix - R) & (RING - 1)) * PL + me;
+#pragma unroll
+    for (int iy = 0; iy < WY; ++iy)
+#pragma unroll
+      for (int iz = 0; iz < W; ++iz) {
+        const double wk = s_wc[(ix * WY + iy) * W + iz];
+        const double v = p[(iy - RY) * PZ + iz - R];
+        sv = wk != 0.0 ? fma(wk, v, sv) : sv;
+      }
+  }
+  return sv;
+}
+
 // MODE 1: forward, 2: transpose (values v / N), 3: SIMP adjoint (values s / N).
-// Tile: TY x TZ outputs per plane (3-D 4 x 64, 2-D 1 x 256), one per thread.  Ring of
+// Tile: TY x TZ outputs per plane (3-D TY x 64, 2-D 1 x 256), one per thread.  Ring of
 // 8 element planes (x mod 8): 2 kMaxR + 1 in use plus the one being filled.  EN (MODE 3, 3-D): 1 = the
 // element energy in the transformed basis (element matrices with the brick's reflection
 // structure, chosen on the host), 2 = the full product; EN + 4: a penalty other than 3.
-template <int MODE, int DIM, int EN>
-__global__ __launch_bounds__(256) void stencil_kernel(const Grid g, const SGrid sg, int n_off,
+template <int MODE, int DIM, int EN, int TY>
+__global__ __launch_bounds__(DIM == 3 ? 64 * TY : 256)
+__attribute__((amdgpu_waves_per_eu(MODE == 3 ? 2 : 4))) void stencil_kernel(const Grid g, const SGrid sg, int cx,
+                                                                          int n_off,
                                                       const int32_t* __restrict__ off,
                                                       const double* __restrict__ w,
                                                       const double* __restrict__ x,
                                                       const AdjArgs adj,
                                                       double* __restrict__ y) {
-  constexpr int TY = DIM == 3 ? 4 : 1, TZ = 256 / TY;
+  static_assert(DIM == 3 || TY == 1, "2-D tiles are one row");
+  constexpr int TZ = DIM == 3 ? 64 : 256, NT = TY * TZ;
   constexpr int RYM = DIM == 3 ? kMaxR : 0;  // y halo capacity (2-D grids have one y row)
   constexpr int PY = TY + 2 * RYM, PZ = TZ + 2 * kMaxR, PL = PY * PZ;
   constexpr int RING = 8;
@@ -277,18 +305,27 @@ __global__ __launch_bounds__(256) void stencil_kernel(const Grid g, const SGrid 
   __shared__ double sK[NK];
   __shared__ double sG[MODE == 3 ? 63 : 1];
   __shared__ int s_r[3];
-  __shared__ double s_wsum;
+  // MODE 1: weight sum of every boundary class.  An output's in-grid taps depend only on its
+  // distances to the six faces clipped to the radius, (lx, hx, ly, hy, lz, hz) in [0, r]^6.
+  constexpr int NB = MODE == 1 ? (DIM == 3 ? 729 : 81) : 1;
+  __shared__ double s_bw[NB];
+  __shared__ int s_d[kMaxTaps];  // dx, dy, dz + 8, 5 bits each
+  constexpr int NC = DIM == 3 ? 125 : 25;  // (2 kMaxR + 1)^DIM cube positions
+  __shared__ double s_wc[NC];
+  __shared__ int s_cube;
   const int tid = threadIdx.x;
   if (tid < 3) s_r[tid] = 0;
+  for (int t = tid; t < NC; t += NT) s_wc[t] = 0.0;
   if (MODE == 3)
-    for (int t = tid; t < NK; t += 256) sK[t] = adj.K0[t];
+    for (int t = tid; t < NK; t += NT) sK[t] = adj.K0[t];
   __syncthreads();
-  for (int t = tid; t < n_off; t += 256) {
+  for (int t = tid; t < n_off; t += NT) {
     const int dx = off[t * DIM];
     const int dy = DIM == 3 ? off[t * 3 + 1] : 0;
     const int dz = DIM == 3 ? off[t * 3 + 2] : off[t * 2 + 1];
     s_tap[t] = ((dx + 8) << 16) | (dy * PZ + dz + (1 << 15));
     s_w[t] = w[t];
+    s_d[t] = ((dx + 8) << 10) | ((dy + 8) << 5) | (dz + 8);
     atomicMax(&s_r[0], abs(dx));
     atomicMax(&s_r[1], abs(dy));
     atomicMax(&s_r[2], abs(dz));
@@ -305,18 +342,51 @@ __global__ __launch_bounds__(256) void stencil_kernel(const Grid g, const SGrid 
       }
     sG[tid] = fk_kept(M, c, d) ? sum / 64.0 : 0.0;
   }
-  if (MODE == 1 && tid == 0) {
-    // weight sum of an element whose taps are all in the grid, in the reference's order
-    double ws = 0.0;
-    for (int t = 0; t < n_off; ++t) ws += s_w[t];
-    s_wsum = ws;
-  }
   __syncthreads();
   const int rx = s_r[0], ry = DIM == 3 ? s_r[1] : 0, rz = s_r[2];
   const bool wide = rx > kMaxR || ry > RYM || rz > kMaxR;
+  // boundary classes: index ((((lx * (rx+1) + hx) * (ry+1) + ly) * (ry+1) + hy) * (rz+1) + lz) * (rz+1) + hz;
+  // each sum adds the in-grid taps in tap order, as the reference's loop does
+  const int bx = rx + 1, by = ry + 1, bz = rz + 1;
+  // cube form of the tap sum: taps strictly ascending in (dx, dy, dz), radius <= kMaxR
+  const int rc = max(rx, max(ry, rz));
+  if (!wide) {
+    const int wc = 2 * rc + 1, wcy = DIM == 3 ? wc : 1, rcy = DIM == 3 ? rc : 0;
+    for (int t = tid; t < n_off; t += NT) {
+      const int d = s_d[t];
+      const int dx = (d >> 10) - 8, dy = ((d >> 5) & 31) - 8, dz = (d & 31) - 8;
+      s_wc[((dx + rc) * wcy + dy + rcy) * wc + dz + rc] = s_w[t];
+    }
+    if (tid == 0) {
+      int ok = 1;
+      for (int t = 1; t < n_off; ++t) ok &= s_d[t] > s_d[t - 1];
+      s_cube = ok;
+    }
+    __syncthreads();
+  }
+  const bool cube = !wide && s_cube != 0;
+  if (MODE == 1 && !wide) {
+    const int nbc = bx * bx * by * by * bz * bz;
+    for (int c = tid; c < nbc; c += NT) {
+      int r = c;
+      const int hz = r % bz; r /= bz;
+      const int lz = r % bz; r /= bz;
+      const int hy = r % by; r /= by;
+      const int ly = r % by; r /= by;
+      const int hx = r % bx, lx = r / bx;
+      double ws = 0.0;
+      for (int t = 0; t < n_off; ++t) {
+        const int d = s_d[t];
+        const int dx = (d >> 10) - 8, dy = ((d >> 5) & 31) - 8, dz = (d & 31) - 8;
+        ws += (dx >= -lx && dx <= hx && dy >= -ly && dy <= hy && dz >= -lz && dz <= hz) ? s_w[t] : 0.0;
+      }
+      s_bw[c] = ws;
+    }
+    __syncthreads();
+  }
 
   const int y0 = blockIdx.x * TY, z0 = blockIdx.y * TZ;
-  const int x0 = blockIdx.z * kCX, x1 = min(x0 + kCX, sg.X);
+  const int x0 = blockIdx.z * cx, x1 = min(x0 + cx, sg.X);
   const int ty = tid / TZ, tz = tid % TZ;
   const int oy = y0 + ty, oz = z0 + tz;
   const bool mine = oy < sg.Y && oz < sg.Z;
@@ -347,72 +417,134 @@ __global__ __launch_bounds__(256) void stencil_kernel(const Grid g, const SGrid 
   // registers (zero outside the grid), put() stores them to ring slot xs mod 8.  The loop
   // fetches plane xo + rx + 1 before it computes plane xo, so the loads are in flight during
   // the taps; the ring's spare slot (8 > 2 rx + 1) means the store needs no extra barrier.
-  constexpr int KP = (PL + 255) / 256;
-  double pre[KP];
-  auto fetch = [&](int xs) {
+  constexpr int KP = (PL + NT - 1) / NT;
+  // MODES 1 and 2: plane xs is fetched into registers PD steps before it is stored to its
+  // ring slot, so PD planes' loads are in flight while the taps run.  MODE 3 computes each
+  // staged value (an element energy) and stores it at once, one at a time: holding a
+  // plane's share across the taps would take most of the register file.  A ring slot is
+  // written one step before it is read and is not in use then (8 slots, at most
+  // 2 rx + 1 = 5 read per step).
+  constexpr bool kDirect = MODE == 3;
+  constexpr int PD = kDirect ? 1 : 4;
+  double pre[PD][KP];
+  // per staged slot q: its element's offset within a plane (-1: outside the grid in y or z)
+  // and its LDS offset (-1: past the tile); both are the same for every plane
+  int qe[KP], ql[KP];
+#pragma unroll
+  for (int q = 0; q < KP; ++q) {
+    const int t = tid + NT * q;
+    const int ly = t / pz, lz = t - ly * pz;
+    const int gy = y0 - ry + ly, gz = z0 - rz + lz;
+    const bool in = t < pn;
+    ql[q] = in ? (ly + RYM - ry) * PZ + lz + kMaxR - rz : -1;
+    qe[q] = in && gy >= 0 && gy < sg.Y && gz >= 0 && gz < sg.Z ? gy * sg.Z + gz : -1;
+  }
+  // the loads are unconditional (slots outside the grid read element 0 and are zeroed
+  // afterwards), so the loop has no branches around them and a plane's loads stay in
+  // flight across the steps until its store
+  // MODE 2 also keeps N of each slot (prn), the quotient v / N is formed at the store
+  double prn[MODE == 2 ? PD : 1][KP];
+  auto fetch = [&](double (&r)[KP], double (&rn)[KP], int xs) {
+    // planes outside the grid read the nearest one (every lane loads, no branch)
+    const int64_t base = (int64_t)min(max(xs, 0), sg.X - 1) * sg.Y * sg.Z;
 #pragma unroll
     for (int q = 0; q < KP; ++q) {
-      const int t = tid + 256 * q;
-      double v = 0.0;
+      const int64_t e = base + max(qe[q], 0);
+      r[q] = x[e];
+      if (MODE == 2) rn[q] = adj.norm[e];
+    }
+  };
+  auto put = [&](const double (&r)[KP], const double (&rn)[KP], int xs) {
+    double* pl = ring + (xs & (RING - 1)) * PL;
+    const bool xin = xs >= 0 && xs < sg.X;
+#pragma unroll
+    for (int q = 0; q < KP; ++q) {
+      const double v = MODE == 2 ? r[q] / rn[q] : r[q];
+      if (ql[q] >= 0) pl[ql[q]] = xin && qe[q] >= 0 ? v : 0.0;
+    }
+  };
+  auto fill = [&](int xs) {
+    double* pl = ring + (xs & (RING - 1)) * PL;
+#pragma unroll 1
+    for (int q = 0; q < KP; ++q) {
+      const int t = tid + NT * q;
       if (t < pn) {
         const int ly = t / pz, lz = t - (t / pz) * pz;
         const int gy = y0 - ry + ly, gz = z0 - rz + lz;
-        // the element matrix / G coefficients are re-read from LDS per element instead of
-        // being held in registers across the loop (they would take most of the register file)
-        if (MODE == 3) asm volatile("" ::: "memory");
+        double v = 0.0;
+        asm volatile("" ::: "memory");  // K0 / G re-read from LDS per element
         if (xs >= 0 && xs < sg.X && gy >= 0 && gy < sg.Y && gz >= 0 && gz < sg.Z)
           v = stencil_value<MODE, DIM, EN>(g, sg, xs, gy, gz, x, adj, sK, sG);
-      }
-      pre[q] = v;
-    }
-  };
-  auto put = [&](int xs) {
-    double* pl = ring + (xs & (RING - 1)) * PL;
-#pragma unroll
-    for (int q = 0; q < KP; ++q) {
-      const int t = tid + 256 * q;
-      if (t < pn) {
-        const int ly = t / pz, lz = t - (t / pz) * pz;
-        pl[(ly + RYM - ry) * PZ + lz + kMaxR - rz] = pre[q];
+        pl[(ly + RYM - ry) * PZ + lz + kMaxR - rz] = v;
       }
     }
   };
-  for (int xs = x0 - rx; xs < x0 + rx; ++xs) {
-    fetch(xs);
-    put(xs);
-  }
-  fetch(x0 + rx);
-  // interior outputs (every tap in the grid) take the precomputed weight sum
-  const bool yz_in = oy >= ry && oy < sg.Y - ry && oz >= rz && oz < sg.Z - rz;
+  // this output's boundary class in y and z (x is added per plane)
+  const int byz = ((min(oy, ry) * by + min(sg.Y - 1 - oy, ry)) * bz + min(oz, rz)) * bz + min(sg.Z - 1 - oz, rz);
   const int me = (ty + RYM) * PZ + tz + kMaxR - (1 << 15);
-  for (int xo = x0; xo < x1; ++xo) {
-    put(xo + rx);
-    __syncthreads();
-    if (xo + 1 < x1) fetch(xo + rx + 1);
-    if (mine) {
-      double sv = 0.0;
+  auto taps = [&](int xo) {
+    if (!mine) return;
+    double sv = 0.0;
+    // the cube weights are re-read from LDS per plane (uniform broadcasts), not held in
+    // registers across the march
+    asm volatile("" ::: "memory");
+    if (cube && rc == 1) {
+      sv = cube_sum<1, DIM, PZ, PL, RING>(ring, s_wc, xo, me + (1 << 15));
+    } else if (cube && rc == 2) {
+      sv = cube_sum<2, DIM, PZ, PL, RING>(ring, s_wc, xo, me + (1 << 15));
+    } else {
       for (int t = 0; t < n_off; ++t) {
         const int tp = s_tap[t];
         const int slot = (xo + (tp >> 16) - 8) & (RING - 1);
         sv = fma(s_w[t], ring[slot * PL + (tp & 0xffff) + me], sv);
       }
-      double out = sv;
-      if (MODE == 1) {
-        double sw;
-        if (yz_in && xo >= rx && xo < sg.X - rx) {
-          sw = s_wsum;
-        } else {
-          sw = 0.0;
-          for (int t = 0; t < n_off; ++t) {
-            const int a = xo + off[t * DIM];
-            const int b = oy + (DIM == 3 ? off[t * 3 + 1] : 0);
-            const int c = oz + (DIM == 3 ? off[t * 3 + 2] : off[t * 2 + 1]);
-            sw += (a >= 0 && a < sg.X && b >= 0 && b < sg.Y && c >= 0 && c < sg.Z) ? s_w[t] : 0.0;
-          }
-        }
-        out = sw > 0.0 ? sv / sw : 0.0;
+    }
+    double out = sv;
+    if (MODE == 1) {
+      const double sw = s_bw[(min(xo, rx) * bx + min(sg.X - 1 - xo, rx)) * (by * by * bz * bz) + byz];
+      out = sw > 0.0 ? sv / sw : 0.0;
+    }
+    y[((int64_t)xo * sg.Y + oy) * sg.Z + oz] = out;
+  };
+  if (kDirect) {
+    for (int xs = x0 - rx; xs <= x0 + rx; ++xs) fill(xs);
+    for (int xo = x0; xo < x1; ++xo) {
+      __syncthreads();
+      if (xo + 1 < x1) fill(xo + rx + 1);
+      taps(xo);
+    }
+  } else {
+    // planes x0 - rx .. x0 + rx - 1 go to the ring now; x0 + rx + k is in flight in pre[k]
+    for (int xs = x0 - rx; xs < x0 + rx; ++xs) {
+      fetch(pre[0], prn[0], xs);
+      put(pre[0], prn[0], xs);
+    }
+#pragma unroll
+    for (int k = 0; k < PD; ++k)
+      if (x0 + k < x1) fetch(pre[k], prn[MODE == 2 ? k : 0], x0 + rx + k);
+    // step xo stores plane xo + rx from pre[k] and refills pre[k] with plane xo + rx + PD.
+    // Whole groups of PD steps run without conditions (planes past the grid are fetched
+    // clamped and never stored), so every path through the loop issues the same loads and
+    // each store waits only for its own plane's loads.
+    int xb = x0;
+    for (; xb + PD <= x1; xb += PD) {
+#pragma unroll
+      for (int k = 0; k < PD; ++k) {
+        const int xo = xb + k;
+        put(pre[k], prn[MODE == 2 ? k : 0], xo + rx);
+        __syncthreads();
+        fetch(pre[k], prn[MODE == 2 ? k : 0], xo + rx + PD);
+        taps(xo);
       }
-      y[((int64_t)xo * sg.Y + oy) * sg.Z + oz] = out;
+    }
+#pragma unroll
+    for (int k = 0; k < PD; ++k) {
+      const int xo = xb + k;
+      if (xo < x1) {
+        put(pre[k], prn[MODE == 2 ? k : 0], xo + rx);
+        __syncthreads();
+        taps(xo);
+      }
     }
   }
 }
@@ -431,16 +563,27 @@ static int launch_gather(const Grid& g, int n_off, const int32_t* off, const dou
   return ET_OK;
 }
 
+// rows per 3-D tile, 4 or 8
+static int g_stencil_ty = 8;
+void set_stencil_rows(int ty) { g_stencil_ty = ty; }
+int get_stencil_rows() { return g_stencil_ty; }
+
 template <int MODE, int EN>
 static int launch_stencil(const Grid& g, int n_off, const int32_t* off, const double* w, const double* x,
                           const AdjArgs& adj, double* y, hipStream_t s) {
   const SGrid sg = sgrid(g);
-  const int ty = g.dim == 3 ? 4 : 1, tz = 256 / ty;
-  const dim3 grid((sg.Y + ty - 1) / ty, (sg.Z + tz - 1) / tz, (sg.X + kCX - 1) / kCX);
-  if (g.dim == 3)
-    stencil_kernel<MODE, 3, EN><<<grid, 256, 0, s>>>(g, sg, n_off, off, w, x, adj, y);
+  const int ty = g.dim == 3 ? g_stencil_ty : 1, tz = g.dim == 3 ? 64 : 256;
+  const int64_t tiles = (int64_t)((sg.Y + ty - 1) / ty) * ((sg.Z + tz - 1) / tz);
+  // the longest x chunk that still gives about 512 workgroups (two per compute unit)
+  int cx = kMaxCX;
+  while (cx > 4 && tiles * ((sg.X + cx - 1) / cx) < 512) cx /= 2;
+  const dim3 grid((sg.Y + ty - 1) / ty, (sg.Z + tz - 1) / tz, (sg.X + cx - 1) / cx);
+  if (g.dim == 2)
+    stencil_kernel<MODE, 2, EN, 1><<<grid, 256, 0, s>>>(g, sg, cx, n_off, off, w, x, adj, y);
+  else if (ty == 4)
+    stencil_kernel<MODE, 3, EN, 4><<<grid, 256, 0, s>>>(g, sg, cx, n_off, off, w, x, adj, y);
   else
-    stencil_kernel<MODE, 2, EN><<<grid, 256, 0, s>>>(g, sg, n_off, off, w, x, adj, y);
+    stencil_kernel<MODE, 3, EN, 8><<<grid, 512, 0, s>>>(g, sg, cx, n_off, off, w, x, adj, y);
   ET_CHECK_LAUNCH();
   return ET_OK;
 }

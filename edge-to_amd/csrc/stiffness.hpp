@@ -121,6 +121,9 @@ int stencil_from_operator(const Grid& g, const double* K0, const double* E, cons
 // filter
 int filter_normalizer(const Grid& g, int n_off, const int32_t* off, const double* w, double* norm,
                       hipStream_t s);
+// rows of the 3-D filter stencils' tiles (4 or 8)
+void set_stencil_rows(int ty);
+int get_stencil_rows();
 int filter_fwd(const Grid& g, int n_off, const int32_t* off, const double* w, const double* x,
                double* y, hipStream_t s);
 int filter_adj(const Grid& g, int n_off, const int32_t* off, const double* w, const double* norm,

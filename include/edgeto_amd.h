@@ -79,7 +79,9 @@ const char* et_version(void);
  * ET_TUNE_FUSED_RZ: 1 (default) = in PCG with the transformed-basis operator, the
  *   last fine-grid sweep of each V-cycle also sums r.z per workgroup, r taken in the
  *   V-cycle's precision (FP32 V-cycles: their FP32 copy of r); no separate pass over r
- *   and z.  0 = separate reduction with r in FP64.  Read by each solve. */
+ *   and z.  0 = separate reduction with r in FP64.  Read by each solve.
+ * ET_TUNE_FILTER_ROWS: y rows of the 3-D filter stencils' tiles (64 elements along z each):
+ *   8 (default, 512 threads) or 4 (256 threads). */
 enum {
   ET_TUNE_TILED_JC = 0,
   ET_TUNE_MFMA_JC = 1,
@@ -97,7 +99,8 @@ enum {
   ET_TUNE_LEVEL1_MF = 13,
   ET_TUNE_FUSED_RZ = 14,
   ET_TUNE_WHT_SLICES_F32 = 15,
-  ET_TUNE_WHT_SLICES_F64 = 16
+  ET_TUNE_WHT_SLICES_F64 = 16,
+  ET_TUNE_FILTER_ROWS = 17
 };
 int et_tune_set(int param, int value);
 int et_tune_get(int param);

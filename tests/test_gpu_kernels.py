@@ -160,6 +160,44 @@ def test_filter(nel, rmin, spacing):
     assert rel(y.cpu().numpy(), ofilt.transpose(x, nel, offs, w, norm)) < 1e-12
 
 
+@pytest.mark.parametrize("nel,rmin,spacing", [((70, 9, 66), 2.0, (1, 1, 1)), ((17, 11, 130), 2.6, (1, 1, 1)),
+                                               ((10, 6, 4), 2.2, (1.0, 0.5, 2.0)), ((300, 41), 1.5, (1, 1))])
+def test_filter_tap_orders_and_tiles(nel, rmin, spacing):
+    """The stencils sum an offset table in ascending (dx, dy, dz) order over the unrolled
+    cube of radius <= 2, and any other order with the per-tap loop; both agree with the
+    oracle, and 4- and 8-row tiles give the same bits."""
+    L = _lib()
+    dim = len(nel)
+    offs, w = ofilt.offsets_weights(rmin, spacing)
+    norm = ofilt.normalizer(nel, offs, w)
+    n = int(np.prod(nel))
+    x = dev(np.random.default_rng(12).uniform(-1, 1, n))
+    nt = dev(norm)
+    perm = np.random.default_rng(13).permutation(len(w))
+    ref_f = ofilt.forward(x.cpu().numpy(), nel, offs, w)
+    ref_t = ofilt.transpose(x.cpu().numpy(), nel, offs, w, norm)
+    outs = {}
+    try:
+        for rows in (4, 8):
+            L.call("et_tune_set", 17, rows)
+            for order, (o, ww) in (("asc", (offs, w)), ("perm", (offs[perm], w[perm]))):
+                o_t, w_t = dev(np.ascontiguousarray(o).astype(np.int32)), dev(np.ascontiguousarray(ww))
+                yf = torch.empty(n, dtype=torch.float64, device="cuda")
+                yt = torch.empty(n, dtype=torch.float64, device="cuda")
+                L.call("et_filter_fwd_f64", dim, L.nel_arg(nel), len(ww), L.ptr(o_t), L.ptr(w_t), L.ptr(x),
+                       L.ptr(yf), L.stream())
+                L.call("et_filter_adj_f64", dim, L.nel_arg(nel), len(ww), L.ptr(o_t), L.ptr(w_t), L.ptr(nt),
+                       L.ptr(x), L.ptr(yt), L.stream())
+                assert rel(yf.cpu().numpy(), ref_f) < 1e-12, (rows, order)
+                assert rel(yt.cpu().numpy(), ref_t) < 1e-12, (rows, order)
+                outs[(rows, order)] = (yf, yt)
+    finally:
+        L.call("et_tune_set", 17, 8)
+    for order in ("asc", "perm"):
+        assert torch.equal(outs[(4, order)][0], outs[(8, order)][0])
+        assert torch.equal(outs[(4, order)][1], outs[(8, order)][1])
+
+
 @pytest.mark.parametrize("nel", [(8, 4), (12, 6), (8, 4, 4), (6, 4, 8), (7, 5, 3)])
 def test_transfers(nel):
     L = _lib()
