@@ -7,6 +7,7 @@
 // dense symmetric product per coarse solve for the last two).
 // The matrix is padded to a multiple of 64 with an identity block.
 #include <algorithm>
+#include <type_traits>
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -348,54 +349,56 @@ __global__ __launch_bounds__(256) void lower_transpose_kernel(const double* __re
 }
 
 // t_i = sum_{j <= i} Y_ij b_j  (row i of Y = row i of Z); one wavefront per row
-template <typename TI>
-__global__ __launch_bounds__(256) void trmv_lower_kernel(const double* __restrict__ Z, int ld, int n,
+template <typename TM, typename TI>
+__global__ __launch_bounds__(256) void trmv_lower_kernel(const TM* __restrict__ Z, int ld, int n,
                                                          const TI* __restrict__ b, double* __restrict__ t) {
+  using V2 = typename std::conditional<sizeof(TM) == 8, double2, float2>::type;
   const int i = n - 1 - (blockIdx.x * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (i < 0) return;
-  const double* z = Z + (int64_t)i * ld;
-  // pairs of consecutive values per lane (16-byte loads; rows start 16-byte aligned, ld even)
+  const TM* z = Z + (int64_t)i * ld;
+  // pairs of consecutive values per lane (rows start aligned to the pair, ld even)
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   int j = 2 * lane;
   for (; j + 257 <= i; j += 256) {
-    const double2 za = *reinterpret_cast<const double2*>(z + j);
-    const double2 zb = *reinterpret_cast<const double2*>(z + j + 128);
+    const V2 za = *reinterpret_cast<const V2*>(z + j);
+    const V2 zb = *reinterpret_cast<const V2*>(z + j + 128);
     s0 += za.x * (double)b[j];
     s1 += za.y * (double)b[j + 1];
     s2 += zb.x * (double)b[j + 128];
     s3 += zb.y * (double)b[j + 129];
   }
   for (; j <= i; j += 128) {
-    s0 += z[j] * (double)b[j];
-    if (j + 1 <= i) s1 += z[j + 1] * (double)b[j + 1];
+    s0 += (double)z[j] * (double)b[j];
+    if (j + 1 <= i) s1 += (double)z[j + 1] * (double)b[j + 1];
   }
   const double s = wave_sum((s0 + s1) + (s2 + s3));
   if (lane == 0) t[i] = s;
 }
 
 // x_j = sum_{i >= j} Y_ij t_i  (column j of Y); one wavefront per column
-template <typename TO>
-__global__ __launch_bounds__(256) void trmv_upper_kernel(const double* __restrict__ Y, int ld, int n,
+template <typename TM, typename TO>
+__global__ __launch_bounds__(256) void trmv_upper_kernel(const TM* __restrict__ Y, int ld, int n,
                                                          const double* __restrict__ t, TO* __restrict__ x) {
+  using V2 = typename std::conditional<sizeof(TM) == 8, double2, float2>::type;
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (j >= n) return;
-  const double* y = Y + (int64_t)j * ld;
+  const TM* y = Y + (int64_t)j * ld;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   // start at the even index at or below j; the entry above the diagonal (i = j - 1) is zero
   int i = (j & ~1) + 2 * lane;
   for (; i + 129 < n; i += 256) {
-    const double2 ya = *reinterpret_cast<const double2*>(y + i);
-    const double2 yb = *reinterpret_cast<const double2*>(y + i + 128);
+    const V2 ya = *reinterpret_cast<const V2*>(y + i);
+    const V2 yb = *reinterpret_cast<const V2*>(y + i + 128);
     s0 += ya.x * t[i];
     s1 += ya.y * t[i + 1];
     s2 += yb.x * t[i + 128];
     s3 += yb.y * t[i + 129];
   }
   for (; i < n; i += 128) {
-    s0 += y[i] * t[i];
-    if (i + 1 < n) s1 += y[i + 1] * t[i + 1];
+    s0 += (double)y[i] * t[i];
+    if (i + 1 < n) s1 += (double)y[i + 1] * t[i + 1];
   }
   const double s = wave_sum((s0 + s1) + (s2 + s3));
   if (lane == 0) x[j] = (TO)s;
@@ -429,7 +432,12 @@ static int g_coarse_lib = 2;
 void set_coarse_lib(int mode) { g_coarse_lib = mode >= 0 && mode <= 2 ? mode : 2; }
 int get_coarse_lib() { return g_coarse_lib; }
 
-int CoarseSolver::init(int n_, hipStream_t s) {
+__global__ void f64_to_f32_kernel(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = (float)a[t];
+}
+
+int CoarseSolver::init(int n_, hipStream_t s, bool f32_copy) {
   n = n_;
   ld = (n + NB - 1) / NB * NB;
   nbt = ld / NB;
@@ -442,6 +450,10 @@ int CoarseSolver::init(int n_, hipStream_t s) {
     ET_CHECK_HIP(hipMalloc(&Z, bytes));
     ET_CHECK_HIP(hipMalloc(&Wd, (size_t)NB * NB * sizeof(double)));
     ET_CHECK_HIP(hipMalloc(&tv, (size_t)ld * sizeof(double)));
+    if (f32_copy) {
+      ET_CHECK_HIP(hipMalloc(&Zf, (size_t)ld * ld * sizeof(float)));
+      ET_CHECK_HIP(hipMalloc(&Af, (size_t)ld * ld * sizeof(float)));
+    }
   }
   ET_CHECK_HIP(hipMalloc(&info, 2 * sizeof(int)));
   ET_CHECK_HIP(hipHostMalloc(&info_h, 2 * sizeof(int)));
@@ -462,7 +474,10 @@ void CoarseSolver::release() {
   if (Wd) (void)hipFree(Wd);
   if (tv) (void)hipFree(tv);
   if (Z) (void)hipFree(Z);
+  if (Zf) (void)hipFree(Zf);
+  if (Af) (void)hipFree(Af);
   Wd = tv = Z = nullptr;
+  Zf = Af = nullptr;
   if (info) (void)hipFree(info);
   if (info_h) (void)hipHostFree(info_h);
   A = Y = nullptr;
@@ -510,6 +525,12 @@ int CoarseSolver::factor(const Grid& g, int dof, const double* S, hipStream_t s)
                "rocsolver_dtrtri failed");
     lower_transpose_kernel<<<nbt * (nbt + 1) / 2, 256, 0, s>>>(A, ld, Z);
     ET_CHECK_LAUNCH();
+    if (Zf != nullptr) {
+      const int64_t cnt = (int64_t)ld * ld;
+      f64_to_f32_kernel<<<grid_for(cnt), kBlock, 0, s>>>(cnt, A, Af);
+      f64_to_f32_kernel<<<grid_for(cnt), kBlock, 0, s>>>(cnt, Z, Zf);
+      ET_CHECK_LAUNCH();
+    }
     ET_CHECK_HIP(hipMemcpyAsync(info_h, info, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     return ET_OK;
   }
@@ -564,9 +585,16 @@ int CoarseSolver::check_info(hipStream_t s) {
 template <typename TI, typename TO>
 int CoarseSolver::solve(const TI* b, TO* x, hipStream_t s) const {
   if (lib_mode == 2) {
-    trmv_lower_kernel<TI><<<(n + 3) / 4, 256, 0, s>>>(Z, ld, n, b, tv);
-    ET_CHECK_LAUNCH();
-    trmv_upper_kernel<TO><<<(n + 3) / 4, 256, 0, s>>>(A, ld, n, tv, x);
+    if (sizeof(TO) == 4 && Zf != nullptr) {
+      // FP32 V-cycles apply the FP32 copy of the factors (half the bytes), FP64 sums
+      trmv_lower_kernel<float, TI><<<(n + 3) / 4, 256, 0, s>>>(Zf, ld, n, b, tv);
+      ET_CHECK_LAUNCH();
+      trmv_upper_kernel<float, TO><<<(n + 3) / 4, 256, 0, s>>>(Af, ld, n, tv, x);
+    } else {
+      trmv_lower_kernel<double, TI><<<(n + 3) / 4, 256, 0, s>>>(Z, ld, n, b, tv);
+      ET_CHECK_LAUNCH();
+      trmv_upper_kernel<double, TO><<<(n + 3) / 4, 256, 0, s>>>(A, ld, n, tv, x);
+    }
     ET_CHECK_LAUNCH();
     return ET_OK;
   }

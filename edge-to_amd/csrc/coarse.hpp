@@ -17,9 +17,13 @@ struct CoarseSolver {
   double* Wd = nullptr;   // lib_mode 2: inverse of the current diagonal factor tile (64x64)
   double* tv = nullptr;   // lib_mode 2: intermediate vector of the coarse solve
   double* Z = nullptr;    // lib_mode 2: L^-1 row-major (A holds it column-major)
+  float* Zf = nullptr;    // lib_mode 2 with an FP32 copy: Z and A rounded to FP32, applied by FP32 solves
+  float* Af = nullptr;
   void* blas = nullptr;   // rocblas_handle
 
-  int init(int n, hipStream_t s);
+  // f32_copy: FP32 solves (mixed-precision V-cycles) apply FP32 copies of the
+  // FP64 factors (lib_mode 2)
+  int init(int n, hipStream_t s, bool f32_copy = false);
   void release();
   // Expand the block stencil S on grid g, factor, and form the inverse.
   int factor(const Grid& g, int dof, const double* S, hipStream_t s);

@@ -80,7 +80,8 @@ __host__ inline size_t wht_lds_values(const Grid& g, int ns) {
          + 6 * (size_t)tc * nz    // node contributions [3][2][tc][nz]
          + 6 * (size_t)tc * 2     // seam slots [3][2][tc][2]
          + (size_t)tc * g.ez      // E of one layer
-         + 64;                    // G
+         + 64                     // G
+         + 576;                   // K0, staged once to form G
 }
 
 // lane j takes lane j-1 of the wavefront, lane 0 takes 0
@@ -176,7 +177,12 @@ void op0_wht_kernel(const MxArgs A, const T* __restrict__ K0,
   constexpr int kWEStage = Cfg::estage;
   constexpr int kWNodes = Cfg::nodes;
 
-  // G[M][c][d] from K0 (corner order of K0 is the H8 local order)
+  // G[M][c][d] from K0 (corner order of K0 is the H8 local order).  K0 is staged
+  // in LDS by the whole workgroup first (one coalesced load instead of a chain of
+  // dependent loads per thread).
+  T* sk0 = sg + 64;
+  for (int t = tid; t < 576; t += NT) sk0[t] = K0[t];
+  __syncthreads();
   if (tid < kWG) {
     const int M = 1 + tid / 9, c = (tid / 3) % 3, d = tid % 3;
     const int Mp = M ^ (1 << c) ^ (1 << d);
@@ -186,7 +192,7 @@ void op0_wht_kernel(const MxArgs A, const T* __restrict__ K0,
       for (int bp = 0; bp < 8; ++bp) {
         const int ap = h8_corner(bp & 1, (bp >> 1) & 1, bp >> 2);
         const int sg1 = __builtin_popcount(M & bb) & 1, sg2 = __builtin_popcount(Mp & bp) & 1;
-        const double k = (double)K0[(3 * a + c) * 24 + 3 * ap + d];
+        const double k = (double)sk0[(3 * a + c) * 24 + 3 * ap + d];
         s += (sg1 ^ sg2) ? -k : k;
       }
     }
@@ -324,13 +330,14 @@ void op0_wht_kernel(const MxArgs A, const T* __restrict__ K0,
       }
     };
 
-    // prologue: node planes ja-1 and ja, element layer ja-1
+    // prologue: node planes ja-1 and ja, element layer ja-1 (the layer's E is
+    // requested with the first plane)
     fetch_u(ja - 1);
+    fetch_e(ja - 1);
     store_u(ja - 1);
+    store_e();
     fetch_u(ja);
     store_u(ja);
-    fetch_e(ja - 1);
-    store_e();
     __syncthreads();
 
     // FP32 carries the transformed bottom face from the previous layer; FP64

@@ -750,7 +750,7 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
     ET_TRY(h64.alloc(g0, dof, p.n_level, s, use_l1));
   else
     ET_TRY(h32.alloc(g0, dof, p.n_level, s, use_l1));
-  ET_TRY(coarse.init((int)(coarse_grid.n_nodes() * dof), s));
+  ET_TRY(coarse.init((int)(coarse_grid.n_nodes() * dof), s, p.precision == 1));
   h64.coarse = &coarse;
   h32.coarse = &coarse;
   for (auto* h : {&h64.n_smooth, &h32.n_smooth}) *h = p.n_smooth;
