@@ -16,7 +16,7 @@ OBJ_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libedgeto_amd.so")
 
 SOURCES = ["stiffness.hip", "filter.hip", "mg.hip", "coarse.hip", "blas.hip", "oc.hip",
-           "solver.hip", "op_tiled.hip", "galerkin_elem.hip", "op_mfma.hip", "op_matrix.hip", "op_wht.hip", "level1.hip", "k0_structure.cpp", "fea.hip", "krylov.hip", "batch.cpp", "multimat.hip", "capi.cpp"]
+           "solver.hip", "op_tiled.hip", "galerkin_elem.hip", "op_mfma.hip", "op_matrix.hip", "op_wht.hip", "level1.hip", "k0_structure.cpp", "fea.hip", "krylov.hip", "batch.cpp", "multimat.hip", "unstructured.hip", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

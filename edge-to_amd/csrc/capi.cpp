@@ -490,6 +490,74 @@ int et_oc_update_f64(int64_t n, const double* x, const double* dc, double A, dou
   ET_GUARD({ return et::oc_update(n, x, dc, A, vol, vf, move, x_new, work, steps, stream); })
 }
 
+int et_oc_update_elem_f64(int64_t n, const double* x, const double* dc, const double* A_elem, double vol,
+                          double vf, double move, double* x_new, void* work, int* steps, hipStream_t stream) {
+  ET_GUARD({
+    ET_REQUIRE(A_elem != nullptr, "et_oc_update_elem_f64: A_elem is null");
+    return et::oc_update(n, x, dc, 0.0, vol, vf, move, x_new, work, steps, stream, A_elem);
+  })
+}
+
+// ------------------------------------------------------------ general meshes
+
+int et_umesh_star(int64_t n_nodes, int64_t n_el, const int* elem_ptr, const int* elem_nodes, int* star_ptr,
+                  int* star, int* el_of, hipStream_t stream) {
+  ET_GUARD({ return et::umesh_star(n_nodes, n_el, elem_ptr, elem_nodes, star_ptr, star, el_of, stream); })
+}
+
+int et_umatvec_f64(const et_umesh* mesh, const double* E, const uint8_t* cons, const double* u, double* y,
+                   hipStream_t stream) {
+  ET_GUARD({ return et::umatvec(mesh, 0, E, cons, u, nullptr, y, stream); })
+}
+
+int et_udiagonal_f64(const et_umesh* mesh, const double* E, const uint8_t* cons, double* diag,
+                     hipStream_t stream) {
+  ET_GUARD({ return et::udiagonal(mesh, E, cons, diag, stream); })
+}
+
+int et_uelem_energy_f64(const et_umesh* mesh, const double* U, double* dk, hipStream_t stream) {
+  ET_GUARD({ return et::uenergy(mesh, U, nullptr, 0.0, 0.0, 0, dk, stream); })
+}
+
+int et_usensitivity_f64(const et_umesh* mesh, const double* U, const double* xt, double penal, double void_,
+                        double* out, hipStream_t stream) {
+  ET_GUARD({ return et::uenergy(mesh, U, xt, penal, void_, 1, out, stream); })
+}
+
+int et_ufea_integrals_f64(const et_umesh* mesh, int nb, const double* B, const int64_t* B_ptr, const double* D,
+                          const int64_t* D_ptr, const double* E, const double* U, double* strain, double* stress,
+                          double* energy, double* vm, hipStream_t stream) {
+  ET_GUARD({ return et::ufea(mesh, nb, B, B_ptr, D, D_ptr, E, U, strain, stress, energy, vm, stream); })
+}
+
+int et_csr_spmv_f64(int64_t n_rows, const int* row_ptr, const int* cols, const double* vals, const double* x,
+                    double* y, hipStream_t stream) {
+  ET_GUARD({ return et::csr_spmv(n_rows, row_ptr, cols, vals, x, y, stream); })
+}
+
+int et_ucg_f64(const et_umesh* mesh, const double* E, const uint8_t* cons, const double* b, double* x, double tol,
+               int maxiter, void* work, int64_t work_bytes, int* iters, double* residual, hipStream_t stream) {
+  ET_GUARD({
+    ET_REQUIRE(mesh && E && b && x && work, "et_ucg_f64: null pointer");
+    ET_REQUIRE(b != x, "et_ucg_f64: b and x must not alias");
+    ET_REQUIRE(std::isfinite(tol) && tol >= 0, "et_ucg_f64: tol must be finite and >= 0");
+    return et::cg_solve_op(et::unstructured_op(mesh, E, cons), b, x, tol, maxiter, work, work_bytes, iters,
+                           residual, stream);
+  })
+}
+
+int et_ugmres_f64(const et_umesh* mesh, const double* E, const uint8_t* cons, const double* b, double* x,
+                  double tol, int restart, int maxiter, void* work, int64_t work_bytes, int* iters,
+                  double* residual, hipStream_t stream) {
+  ET_GUARD({
+    ET_REQUIRE(mesh && E && b && x && work, "et_ugmres_f64: null pointer");
+    ET_REQUIRE(b != x, "et_ugmres_f64: b and x must not alias");
+    ET_REQUIRE(std::isfinite(tol) && tol >= 0, "et_ugmres_f64: tol must be finite and >= 0");
+    return et::gmres_solve_op(et::unstructured_op(mesh, E, cons), b, x, tol, restart, maxiter, work, work_bytes,
+                              iters, residual, stream);
+  })
+}
+
 int et_oc_update_f64_noalloc(int64_t n, const double* x, const double* dc, double A, double vol,
                              double vf, double move, double* x_new, int* steps, hipStream_t stream) {
   ET_GUARD({ return et::oc_update_noalloc(n, x, dc, A, vol, vf, move, x_new, steps, stream); })

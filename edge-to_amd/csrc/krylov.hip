@@ -1,7 +1,7 @@
-// Unpreconditioned Krylov solvers on the matrix-free structured operator:
-// conjugate gradients and restarted GMRES, the iterations of CuCG / CuGMRES
-// (solvers/_gpu.py:11-107), which hand the stiffness operator to
-// cupyx.scipy.sparse.linalg.cg / gmres.
+// Unpreconditioned Krylov solvers on a matrix-free operator (the structured
+// grid operator or the general-mesh operator, KrylovOp): conjugate gradients
+// and restarted GMRES, the iterations of CuCG / CuGMRES (solvers/_gpu.py:11-107),
+// which hand the stiffness operator to cupyx.scipy.sparse.linalg.cg / gmres.
 //
 // CG (cupyx cg semantics): atol = tol |b|; rho = r.r, beta = rho / rho_old,
 // p = r + beta p, q = A p, alpha = rho / p.q, x += alpha p, r -= alpha q,
@@ -151,10 +151,20 @@ struct Carve {
   }
 };
 
-int residual_norm(const Grid& g, const double* K0, const double* E, const uint8_t* cons, bool fine,
-                  const double* b, const double* x, double* r, double* partial, double* out, hipStream_t s) {
-  ET_TRY(level0_op<double>(1, g, K0, E, cons, x, b, nullptr, nullptr, 1.0, nullptr, r, s, fine));
-  return dot(g.n_nodes() * g.dim, r, r, partial, out, s);
+int residual_norm(const KrylovOp& op, const double* b, const double* x, double* r, double* partial, double* out,
+                  hipStream_t s) {
+  ET_TRY(op.apply(1, x, b, r, s));
+  return dot(op.n, r, r, partial, out, s);
+}
+
+// the structured operator (fine-grid kernels when `fine`) as a Krylov operator
+KrylovOp structured_op(const Grid& g, const double* K0, const double* E, const uint8_t* cons, bool fine) {
+  KrylovOp op;
+  op.n = g.n_nodes() * g.dim;
+  op.apply = [=](int mode, const double* x, const double* b, double* y, hipStream_t s) {
+    return level0_op<double>(mode, g, K0, E, cons, x, b, nullptr, nullptr, 1.0, nullptr, y, s, fine);
+  };
+  return op;
 }
 
 }  // namespace
@@ -162,7 +172,13 @@ int residual_norm(const Grid& g, const double* K0, const double* E, const uint8_
 int cg_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, bool fine, const double* b,
              double* x, double tol, int maxiter, void* work, int64_t work_bytes, int* iters, double* residual,
              hipStream_t s) {
-  const int64_t n = g.n_nodes() * g.dim;
+  return cg_solve_op(structured_op(g, K0, E, cons, fine), b, x, tol, maxiter, work, work_bytes, iters, residual,
+                     s);
+}
+
+int cg_solve_op(const KrylovOp& op, const double* b, double* x, double tol, int maxiter, void* work,
+                int64_t work_bytes, int* iters, double* residual, hipStream_t s) {
+  const int64_t n = op.n;
   ET_REQUIRE(maxiter >= 1, "cg: maxiter must be >= 1");
   ET_REQUIRE(work_bytes >= krylov_work_size(n, 0), "cg: work buffer too small");
   Carve c{static_cast<char*>(work)};
@@ -174,7 +190,7 @@ int cg_solve(const Grid& g, const double* K0, const double* E, const uint8_t* co
   CgState* st = c.take<CgState>(1);
   double* scal = c.take<double>(4);  // [0] |b|^2, [1] |b - A x|^2 after the loop
   // r = b - A x, |b|^2, |r|^2
-  ET_TRY(level0_op<double>(1, g, K0, E, cons, x, b, nullptr, nullptr, 1.0, nullptr, r, s, fine));
+  ET_TRY(op.apply(1, x, b, r, s));
   ET_TRY(dot_partials(n, b, b, part_a, s));
   ET_TRY(dot_partials(n, r, r, part_b, s));
   cg_init_kernel<<<1, 256, 0, s>>>(part_a, part_b, kRedBlocks, tol, st);
@@ -184,7 +200,7 @@ int cg_solve(const Grid& g, const double* K0, const double* E, const uint8_t* co
     const int chunk = std::min(kCgChunk, maxiter - done);
     for (int k = 0; k < chunk; ++k) {
       cg_p_kernel<<<kRedBlocks, 256, 0, s>>>(n, st, r, p);
-      ET_TRY(level0_op<double>(0, g, K0, E, cons, p, nullptr, nullptr, nullptr, 1.0, nullptr, q, s, fine));
+      ET_TRY(op.apply(0, p, nullptr, q, s));
       cg_pq_kernel<<<kRedBlocks, 256, 0, s>>>(n, st, p, q, part_a);
       cg_xr_kernel<<<kRedBlocks, 256, 0, s>>>(n, st, part_a, kRedBlocks, p, q, x, r, part_b);
       cg_check_kernel<<<1, 256, 0, s>>>(part_b, kRedBlocks, maxiter, st);
@@ -198,7 +214,7 @@ int cg_solve(const Grid& g, const double* K0, const double* E, const uint8_t* co
   ET_CHECK_HIP(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
   // the reported residual is the true one, |b - A x| / |b| (solvers/_gpu.py:43)
   ET_TRY(dot(n, b, b, part_a, scal, s));
-  ET_TRY(residual_norm(g, K0, E, cons, fine, b, x, r, part_b, scal + 1, s));
+  ET_TRY(residual_norm(op, b, x, r, part_b, scal + 1, s));
   double tail[2];
   ET_CHECK_HIP(hipMemcpyAsync(tail, scal, sizeof(tail), hipMemcpyDeviceToHost, s));
   ET_CHECK_HIP(hipStreamSynchronize(s));
@@ -303,7 +319,13 @@ static void hessenberg_lstsq(int m, std::vector<double> H, double beta, double* 
 int gmres_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, bool fine,
                 const double* b, double* x, double tol, int restart, int maxiter, void* work, int64_t work_bytes,
                 int* iters, double* residual, hipStream_t s) {
-  const int64_t n = g.n_nodes() * g.dim;
+  return gmres_solve_op(structured_op(g, K0, E, cons, fine), b, x, tol, restart, maxiter, work, work_bytes, iters,
+                        residual, s);
+}
+
+int gmres_solve_op(const KrylovOp& op, const double* b, double* x, double tol, int restart, int maxiter,
+                   void* work, int64_t work_bytes, int* iters, double* residual, hipStream_t s) {
+  const int64_t n = op.n;
   ET_REQUIRE(restart >= 1 && restart <= 256, "gmres: restart must be in [1, 256]");
   ET_REQUIRE(maxiter >= 1, "gmres: maxiter must be >= 1");
   ET_REQUIRE(work_bytes >= krylov_work_size(n, restart), "gmres: work buffer too small");
@@ -327,7 +349,7 @@ int gmres_solve(const Grid& g, const double* K0, const double* E, const uint8_t*
   double rnorm = 0.0;
   const unsigned grid = (unsigned)std::min<int64_t>(kRedBlocks, (n + 255) / 256);
   while (true) {
-    ET_TRY(level0_op<double>(1, g, K0, E, cons, x, b, nullptr, nullptr, 1.0, nullptr, r, s, fine));
+    ET_TRY(op.apply(1, x, b, r, s));
     gm_multidot_kernel<<<dim3(nb, 1), 256, 0, s>>>(n, r, r, partial, nb);
     gm_finalize_kernel<<<1, 256, 0, s>>>(partial, nb, sc + 1, 1, 1);
     ET_CHECK_LAUNCH();
@@ -340,8 +362,7 @@ int gmres_solve(const Grid& g, const double* K0, const double* E, const uint8_t*
     if (bnorm == 0.0 || rnorm <= atol || it >= maxiter) break;
     gm_scale_kernel<<<grid, 256, 0, s>>>(n, r, sc + 1, V);
     for (int j = 0; j < m; ++j) {
-      ET_TRY(level0_op<double>(0, g, K0, E, cons, V + (int64_t)j * n, nullptr, nullptr, nullptr, 1.0, nullptr, u,
-                               s, fine));
+      ET_TRY(op.apply(0, V + (int64_t)j * n, nullptr, u, s));
       // h = V[:, :j+1]^T u into column j of H, then u -= V h
       gm_multidot_kernel<<<dim3(nb, j + 1), 256, 0, s>>>(n, V, u, partial, nb);
       gm_finalize_kernel<<<j + 1, 256, 0, s>>>(partial, nb, Hd + (int64_t)j * (m + 1), 1, 0);

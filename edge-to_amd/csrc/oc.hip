@@ -60,15 +60,17 @@ __device__ __forceinline__ double oc_value(double x, double ocp_raw, bool all_ze
 }
 
 // ocp = x * sqrt(max(-dc/A, 0)) with NaN -> 0 (SingleMaterial.__ocP__,
-// MaterialModels.py:83-90), and per-block sums of |ocp|
+// MaterialModels.py:83-90), and per-block sums of |ocp|.  A: the element measure,
+// Ae[t] per element where Ae is given (general meshes, mesh.As of shape (n, 1)).
 __global__ __launch_bounds__(256) void oc_prep_kernel(int64_t n, const double* __restrict__ x,
                                                       const double* __restrict__ dc, double A,
+                                                      const double* __restrict__ Ae,
                                                       double* __restrict__ ocp,
                                                       double* __restrict__ partial) {
   double s = 0.0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
-    double q = -dc[t] / A;
+    double q = -dc[t] / (Ae != nullptr ? Ae[t] : A);
     q = q > 0.0 ? q : 0.0;
     double v = x[t] * sqrt(q);
     if (v != v) v = 0.0;  // nan_to_num(nan=0)
@@ -96,6 +98,7 @@ __global__ void oc_init_kernel(int nb, const double* __restrict__ partial, OcSta
 // per-block volume sums sum(x_new(m_c) * A) at every midpoint m_c of the pass
 __global__ __launch_bounds__(256) void oc_pass_kernel(int64_t n, const double* __restrict__ x,
                                                       const double* __restrict__ ocp, double A,
+                                                      const double* __restrict__ Ae,
                                                       double move, const OcState* __restrict__ st,
                                                       double* __restrict__ partial, int nb) {
   if (st->done) return;
@@ -107,9 +110,9 @@ __global__ __launch_bounds__(256) void oc_pass_kernel(int64_t n, const double* _
   for (int c = 0; c < kOcCand; ++c) s[c] = 0.0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
-    const double xv = x[t], pv = ocp[t];
+    const double xv = x[t], pv = ocp[t], av = Ae != nullptr ? Ae[t] : A;
 #pragma unroll
-    for (int c = 0; c < kOcCand; ++c) s[c] += oc_value(xv, pv, all_zero, move, m[c]) * A;
+    for (int c = 0; c < kOcCand; ++c) s[c] += oc_value(xv, pv, all_zero, move, m[c]) * av;
   }
 #pragma unroll
   for (int c = 0; c < kOcCand; ++c) {
@@ -171,19 +174,19 @@ int64_t oc_work_size(int64_t n) {
 }
 
 int oc_update(int64_t n, const double* x, const double* dc, double A, double vol, double vf,
-              double move, double* xn, void* work, int* steps, hipStream_t s) {
+              double move, double* xn, void* work, int* steps, hipStream_t s, const double* Ae) {
   ET_REQUIRE(n > 0 && x && dc && xn && work, "oc_update: bad arguments");
-  ET_REQUIRE(A > 0 && vol > 0, "oc_update: element measure and volume must be positive");
+  ET_REQUIRE((Ae != nullptr || A > 0) && vol > 0, "oc_update: element measure and volume must be positive");
   ET_REQUIRE(xn != x && xn != dc, "oc_update: x_new must not alias x or dc");
   ET_REQUIRE((reinterpret_cast<uintptr_t>(work) & 7) == 0, "oc_update: work must be 8-byte aligned");
   OcState* st = static_cast<OcState*>(work);
   double* ocp = reinterpret_cast<double*>(static_cast<char*>(work) + kOcStateBytes);
   double* partial = ocp + n;
   const int nb = (int)std::min<int64_t>(kRedBlocks, (n + 255) / 256);
-  oc_prep_kernel<<<nb, 256, 0, s>>>(n, x, dc, A, ocp, partial);
+  oc_prep_kernel<<<nb, 256, 0, s>>>(n, x, dc, A, Ae, ocp, partial);
   oc_init_kernel<<<1, 256, 0, s>>>(nb, partial, st);
   for (int pass = 0; pass < kOcPasses; ++pass) {
-    oc_pass_kernel<<<nb, 256, 0, s>>>(n, x, ocp, A, move, st, partial, nb);
+    oc_pass_kernel<<<nb, 256, 0, s>>>(n, x, ocp, A, Ae, move, st, partial, nb);
     oc_decide_kernel<<<1, 256, 0, s>>>(partial, nb, vol, vf, st);
   }
   oc_final_kernel<<<grid_for(n), kBlock, 0, s>>>(n, x, ocp, move, st, xn);
@@ -245,7 +248,7 @@ int oc_update_noalloc(int64_t n, const double* x, const double* dc, double A, do
     }
     work = g_scratch_ptr[dev];
   }
-  return oc_update(n, x, dc, A, vol, vf, move, xn, work, steps, s);
+  return oc_update(n, x, dc, A, vol, vf, move, xn, work, steps, s, nullptr);
 }
 
 // sum (a - b)^2, used for the RMS design change of the SIMP loop

@@ -1,6 +1,7 @@
 // Internal declarations shared between the kernel translation units, the
 // solver and the C-ABI layer.
 #pragma once
+#include <functional>
 #include "common.hpp"
 #include "op_tiled.hpp"
 
@@ -157,6 +158,30 @@ int oc_update_multi(int64_t n, int m, const double* x, const double* dc, double 
 
 // unpreconditioned Krylov solvers on the structured operator (krylov.hip)
 int64_t krylov_work_size(int64_t n, int restart);
+// A linear operator for the Krylov solvers: mode 0 y = A x, mode 1 y = b - A x (b may be
+// null in mode 0), on the given stream; n unknowns.
+struct KrylovOp {
+  int64_t n = 0;
+  std::function<int(int mode, const double* x, const double* b, double* y, hipStream_t s)> apply;
+};
+// general (unstructured) meshes (unstructured.hip)
+int umesh_star(int64_t n_nodes, int64_t n_el, const int* elem_ptr, const int* elem_nodes, int* star_ptr, int* star,
+               int* el_of, hipStream_t s);
+int umatvec(const et_umesh* m, int mode, const double* E, const uint8_t* cons, const double* u, const double* b,
+            double* y, hipStream_t s);
+int udiagonal(const et_umesh* m, const double* E, const uint8_t* cons, double* diag, hipStream_t s);
+int uenergy(const et_umesh* m, const double* U, const double* xt, double penal, double vmin, int mode, double* out,
+            hipStream_t s);
+int ufea(const et_umesh* m, int nb, const double* B, const int64_t* B_ptr, const double* D, const int64_t* D_ptr,
+         const double* E, const double* U, double* strain, double* stress, double* energy, double* vm,
+         hipStream_t s);
+int csr_spmv(int64_t n_rows, const int* row_ptr, const int* cols, const double* vals, const double* x, double* y,
+             hipStream_t s);
+KrylovOp unstructured_op(const et_umesh* m, const double* E, const uint8_t* cons);
+int cg_solve_op(const KrylovOp& op, const double* b, double* x, double tol, int maxiter, void* work,
+                int64_t work_bytes, int* iters, double* residual, hipStream_t s);
+int gmres_solve_op(const KrylovOp& op, const double* b, double* x, double tol, int restart, int maxiter,
+                   void* work, int64_t work_bytes, int* iters, double* residual, hipStream_t s);
 int cg_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, bool fine, const double* b,
              double* x, double tol, int maxiter, void* work, int64_t work_bytes, int* iters, double* residual,
              hipStream_t s);
@@ -166,8 +191,9 @@ int gmres_solve(const Grid& g, const double* K0, const double* E, const uint8_t*
 
 // optimality criteria
 int64_t oc_work_size(int64_t n);
+// Ae: per-element measures (general meshes) in place of the scalar A, or null
 int oc_update(int64_t n, const double* x, const double* dc, double A, double vol, double vf,
-              double move, double* xn, void* work, int* steps, hipStream_t s);
+              double move, double* xn, void* work, int* steps, hipStream_t s, const double* Ae = nullptr);
 // oc_update on a work buffer in a static device array (no allocation); n <= oc_noalloc_max_n()
 int oc_update_noalloc(int64_t n, const double* x, const double* dc, double A, double vol, double vf,
                       double move, double* xn, int* steps, hipStream_t s);

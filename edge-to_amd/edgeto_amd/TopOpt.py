@@ -20,6 +20,12 @@ from . import _lib
 logger = logging.getLogger(__name__)
 
 
+def _n_el(kernel):
+    """Element count of a structured or general stiffness kernel."""
+    n = getattr(kernel, "n_el", None)
+    return int(n) if n is not None else int(kernel.elements.shape[0])
+
+
 class TopOpt:
     def __init__(self, mesh, material_model, kernel, solver, filter_kernel=None, max_iter=500,
                  move=0.2, ch_tol=1e-4, fun_tol=1e-6, reuse_sol=True, solve_min_tol=np.inf,
@@ -44,7 +50,7 @@ class TopOpt:
         dev = kernel.constraints.device
         self.f = torch.zeros(kernel.shape[0], dtype=torch.float64, device=dev)
         self.KDTree = None
-        n = kernel.elements.shape[0]
+        n = _n_el(kernel)
         self._oc_work = torch.empty(int(_lib.lib().et_oc_work_size(n)) // 8 + 1, dtype=torch.float64,
                                     device=dev)
         self._scalar = torch.zeros(4, dtype=torch.float64, device=dev)
@@ -123,8 +129,20 @@ class TopOpt:
                            "(rerun this function and more iteration from prior solve will be applied).")
         mesh = self.mesh
         ne = rho_.numel()
-        nb = mesh.B_single.shape[0]
         dev = self.f.device
+        if hasattr(self.kernel, "_um"):
+            # general meshes: per-element B, D and K (FEA_Integrals_..._{full,flat}, :1722-1800)
+            nb = mesh.B_size
+            strain = torch.empty((ne, nb), dtype=torch.float64, device=dev)
+            stress = torch.empty((ne, nb), dtype=torch.float64, device=dev)
+            energy = torch.empty(ne, dtype=torch.float64, device=dev)
+            vm = torch.empty(ne, dtype=torch.float64, device=dev)
+            _lib.call("et_ufea_integrals_f64", ctypes.byref(self.kernel._um), nb, _lib.f64(mesh.B_flat),
+                      _lib.i64(mesh.B_ptr), _lib.f64(mesh.D_flat), _lib.i64(mesh.D_ptr), _lib.f64(rho_, ne),
+                      _lib.f64(U, self.kernel.shape[0]), _lib.f64(strain), _lib.f64(stress), _lib.f64(energy),
+                      _lib.f64(vm), _lib.stream())
+            return strain, stress, energy, vm
+        nb = mesh.B_single.shape[0]
         strain = torch.empty((ne, nb), dtype=torch.float64, device=dev)
         stress = torch.empty((ne, nb), dtype=torch.float64, device=dev)
         energy = torch.empty(ne, dtype=torch.float64, device=dev)
@@ -143,6 +161,9 @@ class TopOpt:
         """One multiplier per material (n_sets = m), TopOpt/_cuda.py:242-280."""
         mm = self.material_model
         m = mm.n_sets
+        if getattr(self.mesh, "A_host", None) is None:
+            raise NotImplementedError("multi-material design updates need a mesh with one element measure "
+                                      "(structured meshes)")
         rho = rho.reshape(-1, m).contiguous()
         df = df.reshape(-1, m).contiguous()
         n = rho.shape[0]
@@ -169,6 +190,13 @@ class TopOpt:
         out = torch.empty_like(rho)
         n = rho.numel()
         # asynchronous: the step count stays in the work buffer (int32 at byte 36)
+        A_elem = getattr(self.mesh, "A_elem", None)
+        if A_elem is not None:
+            # general meshes: one measure per element (mesh.As)
+            _lib.call("et_oc_update_elem_f64", n, _lib.f64(rho), _lib.f64(df, n), _lib.f64(A_elem, n),
+                      float(self.mesh.volume), float(self.material_model.volume_fraction), float(self.move),
+                      _lib.f64(out, n), _lib.f64(self._oc_work), None, _lib.stream())
+            return out
         _lib.call("et_oc_update_f64", n, _lib.f64(rho), _lib.f64(df, n), float(self.mesh.A_host),
                   float(self.mesh.volume), float(self.material_model.volume_fraction), float(self.move),
                   _lib.f64(out, n), _lib.f64(self._oc_work), None, _lib.stream())
@@ -229,7 +257,7 @@ class TopOpt:
         flag = False
         multi = self.material_model.base_properties()["n_sets"] > 1
         if rho_init is None:
-            rho = self.material_model.init_desvars(self.kernel.elements.shape[0])
+            rho = self.material_model.init_desvars(_n_el(self.kernel))
         elif multi:
             # clip(x0, void, 1) per material (the intent of :321-322)
             x0 = torch.as_tensor(rho_init, dtype=torch.float64, device=self.f.device).reshape(-1).contiguous()

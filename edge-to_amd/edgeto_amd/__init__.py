@@ -1,11 +1,14 @@
-"""edgeto_amd -- MI355X-native drop-in for EDGE-TO's structured SIMP hot path.
+"""edgeto_amd -- MI355X-native drop-in for EDGE-TO's SIMP hot path.
 
 The public names are those of the reference package (EDGETO/__init__.py),
-restricted to the structured GPU path:
+restricted to the GPU path:
 
     CuStructuredMesh2D / CuStructuredMesh3D     geom
     CuStructuredFilter2D / CuStructuredFilter3D geom
     CuStructuredStiffnessKernel                 kernels
+    CuGeneralMesh, CuGeneralFilter,
+    CuUniformStiffnessKernel,
+    CuGeneralStiffnessKernel                    general (unstructured meshes)
     CuMultiGrid, CuCG, CuGMRES                  solvers
     SingleMaterial                              MaterialModels
     CuTopOpt                                    TopOpt
@@ -28,6 +31,10 @@ from .solvers import CG, GMRES, CuCG, CuGMRES, CuMultiGrid, MultiGrid, Solver
 from .TopOpt import CuTopOpt, TopOpt
 from .autograd import ComplianceFunction, compliance, energy_compliance
 from .batch import BatchTopOpt, gmg_levels, random_problem
+from .general import (CuGeneralFilter, CuGeneralMesh, CuGeneralStiffnessKernel, CuUniformStiffnessKernel,
+                      GeneralFilter, GeneralMesh, GeneralStiffnessKernel, UniformStiffnessKernel)
+from .physics import (AutoStiffness, element_matrices, tetrahedron_element_stiffness,
+                      triangle_element_stiffness)
 
 _lib.load(require_gpu=False)
 
@@ -38,4 +45,7 @@ __all__ = ["CuStructuredMesh2D", "CuStructuredMesh3D", "CuStructuredFilter2D",
            "TopOpt", "FilterKernel", "StiffnessKernel", "MaterialModel", "Solver",
            "quadrilateral_element_stiffness", "hexahedron_element_stiffness", "compliance",
            "ComplianceFunction", "energy_compliance", "CuCG", "CuGMRES", "CG", "GMRES", "BatchTopOpt",
-           "gmg_levels", "random_problem", "PenalizedMultiMaterial", "PenalizedMultiMaterialRP"]
+           "gmg_levels", "random_problem", "PenalizedMultiMaterial", "PenalizedMultiMaterialRP",
+           "CuGeneralMesh", "GeneralMesh", "CuUniformStiffnessKernel", "CuGeneralStiffnessKernel",
+           "UniformStiffnessKernel", "GeneralStiffnessKernel", "CuGeneralFilter", "GeneralFilter",
+           "AutoStiffness", "element_matrices", "triangle_element_stiffness", "tetrahedron_element_stiffness"]

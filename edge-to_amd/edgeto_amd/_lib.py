@@ -33,6 +33,14 @@ class MGParams(ctypes.Structure):
                 ("w_level_mask", ctypes.c_int), ("precision", ctypes.c_int)]
 
 
+class UMesh(ctypes.Structure):
+    """et_umesh: a general mesh's device arrays (include/edgeto_amd.h)."""
+    _fields_ = [("n_nodes", ctypes.c_int64), ("n_el", ctypes.c_int64), ("dof", ctypes.c_int),
+                ("elem_ptr", ctypes.c_void_p), ("elem_nodes", ctypes.c_void_p), ("K", ctypes.c_void_p),
+                ("K_ptr", ctypes.c_void_p), ("star_ptr", ctypes.c_void_p), ("star", ctypes.c_void_p),
+                ("el_of", ctypes.c_void_p)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _I64 = ctypes.c_int64
@@ -87,6 +95,18 @@ _SIGS = {
     "et_batch_run": [_P, _I, _I, ctypes.POINTER(ctypes.c_double)],
     "et_batch_result": [_P, _I, _P, _P, _P, _P],
     "et_batch_destroy": [_P],
+    "et_oc_update_elem_f64": [_I64, _P, _P, _P, _D, _D, _D, _P, _P, ctypes.POINTER(ctypes.c_int), _P],
+    "et_umesh_star": [_I64, _I64, _P, _P, _P, _P, _P, _P],
+    "et_umatvec_f64": [ctypes.POINTER(UMesh), _P, _P, _P, _P, _P],
+    "et_udiagonal_f64": [ctypes.POINTER(UMesh), _P, _P, _P, _P],
+    "et_uelem_energy_f64": [ctypes.POINTER(UMesh), _P, _P, _P],
+    "et_usensitivity_f64": [ctypes.POINTER(UMesh), _P, _P, _D, _D, _P, _P],
+    "et_ufea_integrals_f64": [ctypes.POINTER(UMesh), _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "et_csr_spmv_f64": [_I64, _P, _P, _P, _P, _P, _P],
+    "et_ucg_f64": [ctypes.POINTER(UMesh), _P, _P, _P, _P, _D, _I, _P, _I64, ctypes.POINTER(ctypes.c_int),
+                   ctypes.POINTER(ctypes.c_double), _P],
+    "et_ugmres_f64": [ctypes.POINTER(UMesh), _P, _P, _P, _P, _D, _I, _I, _P, _I64, ctypes.POINTER(ctypes.c_int),
+                      ctypes.POINTER(ctypes.c_double), _P],
     "et_mg_create": [ctypes.POINTER(_P), ctypes.POINTER(MGParams), ctypes.POINTER(ctypes.c_double), _P],
     "et_mg_destroy": [_P],
     "et_mg_solve": [_P, _P, _P, _P, _P, _D, _I, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)],
@@ -172,6 +192,10 @@ def f32(t, numel=None):
 
 def i32(t, numel=None):
     return ptr(t, torch.int32, numel)
+
+
+def i64(t, numel=None):
+    return ptr(t, torch.int64, numel)
 
 
 def u8(t, numel=None):

@@ -12,6 +12,7 @@ inside the FP64 outer loop.  The reference's 'W' default behaves as a
 V-cycle (it tests for a lowercase 'w', :165); the same rule applies here.
 """
 import ctypes
+import time
 import logging
 
 import numpy as np
@@ -152,6 +153,7 @@ class _Krylov(Solver):
         self.maxiter = maxiter
         self.matrix_free = True
         self.last_iterations = 0
+        self.last_timings = (0.0, 0.0)
         self._work = None
 
     def _call(self, name, rhs, x):
@@ -162,10 +164,20 @@ class _Krylov(Solver):
             self._work = torch.empty(size // 8 + 1, dtype=torch.float64, device=rhs.device)
         it, res = ctypes.c_int(0), ctypes.c_double(0.0)
         extra = (int(self._restart),) if self._restart else ()
-        status = getattr(_lib.lib(), name)(
-            len(k._nel), _lib.nel_arg(k._nel), _lib.f64(k.K_single), _lib.f64(k.rho, k.elements.shape[0]),
-            k._cons_ptr(), _lib.f64(rhs, n), _lib.f64(x, n), float(self.tol), *extra, int(self.maxiter),
-            _lib.f64(self._work), self._work.numel() * 8, ctypes.byref(it), ctypes.byref(res), _lib.stream())
+        t0 = time.perf_counter()
+        if hasattr(k, "_um"):
+            # general meshes: the same iterations on the unstructured operator
+            status = getattr(_lib.lib(), self._uname)(
+                ctypes.byref(k._um), _lib.f64(k.rho, k.n_el), k._cons_ptr(), _lib.f64(rhs, n), _lib.f64(x, n),
+                float(self.tol), *extra, int(self.maxiter), _lib.f64(self._work), self._work.numel() * 8,
+                ctypes.byref(it), ctypes.byref(res), _lib.stream())
+            name = self._uname
+        else:
+            status = getattr(_lib.lib(), name)(
+                len(k._nel), _lib.nel_arg(k._nel), _lib.f64(k.K_single), _lib.f64(k.rho, k.elements.shape[0]),
+                k._cons_ptr(), _lib.f64(rhs, n), _lib.f64(x, n), float(self.tol), *extra, int(self.maxiter),
+                _lib.f64(self._work), self._work.numel() * 8, ctypes.byref(it), ctypes.byref(res), _lib.stream())
+        self.last_timings = (0.0, (time.perf_counter() - t0) * 1e3)
         # maxiter reached above the tolerance is not an error (cupyx reports it in `info` only)
         if status != _lib.ET_OK and status != _lib.ET_ENOCONV:
             _lib.check(name, status)
@@ -196,12 +208,14 @@ class _Krylov(Solver):
 class CG(_Krylov):
     """Conjugate gradients without preconditioning (solvers/_gpu.py:11-56)."""
     _name = "et_cg_f64"
+    _uname = "et_ucg_f64"
 
 
 class GMRES(_Krylov):
     """Restarted GMRES(20) without preconditioning (solvers/_gpu.py:58-103), cupyx's
     default restart length."""
     _name = "et_gmres_f64"
+    _uname = "et_ugmres_f64"
     _restart = 20
 
 

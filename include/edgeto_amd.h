@@ -304,6 +304,73 @@ int et_gmres_f64(int dim, const int* nel, const double* K0, const double* E, con
                  const double* b, double* x, double tol, int restart, int maxiter, void* work,
                  int64_t work_bytes, int* iters, double* residual, hipStream_t stream);
 
+/* ------------------------------------------------------ general meshes --- */
+/* Unstructured meshes with element matrices stored per element and any mix of
+ * element types (T3 / Q4 in 2-D, T4 / H8 in 3-D): CuGeneralMesh and the
+ * Uniform / General stiffness kernels (geom/_mesh.py:146-253,
+ * kernels/_cuda.py:305-815).  All arrays are device memory:
+ *   elem_ptr   n_el + 1 offsets into elem_nodes (the reference's elements_ptr);
+ *   elem_nodes the elements' node lists, concatenated (elements_flat);
+ *   K, K_ptr   row-major (size * dof)^2 element matrices, concatenated, and their
+ *              n_el + 1 offsets (Ks of a uniform mesh is K_ptr[e] = e * (size*dof)^2);
+ *   star_ptr   n_nodes + 1 offsets into star (node_ids, with the end appended);
+ *   star       positions into elem_nodes, ascending within each node: the stable
+ *              argsort of elem_nodes (sorter);
+ *   el_of      the element of each position of elem_nodes (el_ids).
+ * DOFs are interleaved node * dof + d.  Operators follow the structured ones:
+ * constrained rows are identity rows and constrained columns are skipped
+ * (mat_vec_node_basis_parallel_{full,flat}_wcon, core/_templated_cuda.cu:669-780). */
+typedef struct et_umesh {
+  int64_t n_nodes;
+  int64_t n_el;
+  int dof;
+  const int* elem_ptr;
+  const int* elem_nodes;
+  const double* K;
+  const int64_t* K_ptr;
+  const int* star_ptr;
+  const int* star;
+  const int* el_of;
+} et_umesh;
+
+/* Build star_ptr (n_nodes + 1), star and el_of (elem_ptr[n_el] entries each) on the
+ * device from elem_ptr / elem_nodes; node ids outside [0, n_nodes) are ET_EINVAL.
+ * Allocates temporaries and synchronises `stream` (setup; kernels/_cuda.py:320-331). */
+int et_umesh_star(int64_t n_nodes, int64_t n_el, const int* elem_ptr, const int* elem_nodes,
+                  int* star_ptr, int* star, int* el_of, hipStream_t stream);
+/* y = A(E) u (mat_vec_node_basis_parallel_{full,flat}_wcon); cons may be NULL. */
+int et_umatvec_f64(const et_umesh* mesh, const double* E, const uint8_t* cons, const double* u,
+                   double* y, hipStream_t stream);
+/* diag of A(E), 1 on constrained DOFs (get_diagonal_node_basis_{full,flat}, :1116-1213). */
+int et_udiagonal_f64(const et_umesh* mesh, const double* E, const uint8_t* cons, double* diag,
+                     hipStream_t stream);
+/* dk_e = -u_e^T K_e u_e (process_dk_{full,flat}, :196-231). */
+int et_uelem_energy_f64(const et_umesh* mesh, const double* U, double* dk, hipStream_t stream);
+/* p * clip(xt_e, void, 1)^(p-1) * (-u_e^T K_e u_e) (process_dk + SingleMaterial.__backward__). */
+int et_usensitivity_f64(const et_umesh* mesh, const double* U, const double* xt, double penal,
+                        double void_, double* out, hipStream_t stream);
+/* strain_e = B_e u_e, stress_e = D_e strain_e, energy_e = 1/2 E_e u_e^T K_e u_e and von
+ * Mises (FEA_Integrals_node_basis_parallel_{full,flat}, :1722-1800; TopOpt/_cuda.py:78-129).
+ * B / D: row-major nb x (size*dof) and nb x nb per element with offsets B_ptr / D_ptr
+ * (n_el + 1, int64); nb = 3 (2-D) or 6 (3-D); vm may be NULL. */
+int et_ufea_integrals_f64(const et_umesh* mesh, int nb, const double* B, const int64_t* B_ptr,
+                          const double* D, const int64_t* D_ptr, const double* E, const double* U,
+                          double* strain, double* stress, double* energy, double* vm,
+                          hipStream_t stream);
+/* y = A x for a CSR matrix (row_ptr n_rows + 1, int32 columns): the general-mesh density
+ * filter and its transpose (CuGeneralFilter, geom/_filters.py:278-298, as a stored CSR
+ * transpose).  Each row sums in a fixed order. */
+int et_csr_spmv_f64(int64_t n_rows, const int* row_ptr, const int* cols, const double* vals,
+                    const double* x, double* y, hipStream_t stream);
+/* CG / GMRES(restart) on the general-mesh operator: et_cg_f64 / et_gmres_f64 semantics
+ * (CuCG / CuGMRES with a Uniform or General stiffness kernel, solvers/_gpu.py:11-107). */
+int et_ucg_f64(const et_umesh* mesh, const double* E, const uint8_t* cons, const double* b, double* x,
+               double tol, int maxiter, void* work, int64_t work_bytes, int* iters, double* residual,
+               hipStream_t stream);
+int et_ugmres_f64(const et_umesh* mesh, const double* E, const uint8_t* cons, const double* b,
+                  double* x, double tol, int restart, int maxiter, void* work, int64_t work_bytes,
+                  int* iters, double* residual, hipStream_t stream);
+
 /* ------------------------------------------------- optimality criteria --- */
 /* One OC design update (TopOpt/_cuda.py:242-280 with SingleMaterial.__ocP__ /
  * __con__, MaterialModels.py:75-90): bisection on lambda in [1e-9, 1e9] until
@@ -328,6 +395,13 @@ int et_oc_update_f64(int64_t n, const double* x, const double* dc, double A, dou
  * overlap on one GPU share the array and race; each call re-initialises what it
  * reads.  No counterpart in the reference: for callers that cannot allocate
  * (code under graph capture, serving loops, allocation-free harnesses). */
+/* et_oc_update_f64 with one element measure per element (A_elem, n values: a general
+ * mesh's As, geom/_mesh.py:195-214) in place of the scalar A: ocP = x sqrt(-dc / A_e)
+ * and the volume sum(x_new A_e) / vol (MaterialModels.py:75-90). */
+int et_oc_update_elem_f64(int64_t n, const double* x, const double* dc, const double* A_elem,
+                          double vol, double vf, double move, double* x_new, void* work, int* steps,
+                          hipStream_t stream);
+
 int et_oc_update_f64_noalloc(int64_t n, const double* x, const double* dc, double A, double vol,
                              double vf, double move, double* x_new, int* steps, hipStream_t stream);
 int64_t et_oc_noalloc_max_n(void);
