@@ -535,6 +535,39 @@ int et_csr_spmv_f64(int64_t n_rows, const int* row_ptr, const int* cols, const d
   ET_GUARD({ return et::csr_spmv(n_rows, row_ptr, cols, vals, x, y, stream); })
 }
 
+int et_umesh_pattern(const et_umesh* mesh, int* row_ptr, int* cols, int64_t cap, int64_t* nnz, hipStream_t stream) {
+  ET_GUARD({ return et::umesh_pattern(mesh, row_ptr, cols, cap, nnz, stream); })
+}
+
+int et_uassemble_f64(const et_umesh* mesh, const double* E, const uint8_t* cons, const int* row_ptr,
+                     const int* cols, double* vals, hipStream_t stream) {
+  ET_GUARD({ return et::uassemble(mesh, E, cons, row_ptr, cols, vals, stream); })
+}
+
+int et_csr_cg_f64(int64_t n_rows, const int* row_ptr, const int* cols, const double* vals, const double* b,
+                  double* x, double tol, int maxiter, void* work, int64_t work_bytes, int* iters, double* residual,
+                  hipStream_t stream) {
+  ET_GUARD({
+    ET_REQUIRE(n_rows > 0 && row_ptr && cols && vals && b && x && work, "et_csr_cg_f64: bad arguments");
+    ET_REQUIRE(b != x, "et_csr_cg_f64: b and x must not alias");
+    ET_REQUIRE(std::isfinite(tol) && tol >= 0, "et_csr_cg_f64: tol must be finite and >= 0");
+    return et::cg_solve_op(et::csr_op(n_rows, row_ptr, cols, vals), b, x, tol, maxiter, work, work_bytes, iters,
+                           residual, stream);
+  })
+}
+
+int et_csr_gmres_f64(int64_t n_rows, const int* row_ptr, const int* cols, const double* vals, const double* b,
+                     double* x, double tol, int restart, int maxiter, void* work, int64_t work_bytes, int* iters,
+                     double* residual, hipStream_t stream) {
+  ET_GUARD({
+    ET_REQUIRE(n_rows > 0 && row_ptr && cols && vals && b && x && work, "et_csr_gmres_f64: bad arguments");
+    ET_REQUIRE(b != x, "et_csr_gmres_f64: b and x must not alias");
+    ET_REQUIRE(std::isfinite(tol) && tol >= 0, "et_csr_gmres_f64: tol must be finite and >= 0");
+    return et::gmres_solve_op(et::csr_op(n_rows, row_ptr, cols, vals), b, x, tol, restart, maxiter, work,
+                              work_bytes, iters, residual, stream);
+  })
+}
+
 int et_ucg_f64(const et_umesh* mesh, const double* E, const uint8_t* cons, const double* b, double* x, double tol,
                int maxiter, void* work, int64_t work_bytes, int* iters, double* residual, hipStream_t stream) {
   ET_GUARD({

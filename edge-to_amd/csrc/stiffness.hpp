@@ -175,8 +175,13 @@ int uenergy(const et_umesh* m, const double* U, const double* xt, double penal, 
 int ufea(const et_umesh* m, int nb, const double* B, const int64_t* B_ptr, const double* D, const int64_t* D_ptr,
          const double* E, const double* U, double* strain, double* stress, double* energy, double* vm,
          hipStream_t s);
+// mode 0: y = A x; 1: y = b - A x
 int csr_spmv(int64_t n_rows, const int* row_ptr, const int* cols, const double* vals, const double* x, double* y,
-             hipStream_t s);
+             hipStream_t s, int mode = 0, const double* b = nullptr);
+int umesh_pattern(const et_umesh* m, int* row_ptr, int* cols, int64_t cap, int64_t* nnz, hipStream_t s);
+int uassemble(const et_umesh* m, const double* E, const uint8_t* cons, const int* row_ptr, const int* cols,
+              double* vals, hipStream_t s);
+KrylovOp csr_op(int64_t n_rows, const int* row_ptr, const int* cols, const double* vals);
 KrylovOp unstructured_op(const et_umesh* m, const double* E, const uint8_t* cons);
 int cg_solve_op(const KrylovOp& op, const double* b, double* x, double tol, int maxiter, void* work,
                 int64_t work_bytes, int* iters, double* residual, hipStream_t s);

@@ -103,6 +103,12 @@ _SIGS = {
     "et_usensitivity_f64": [ctypes.POINTER(UMesh), _P, _P, _D, _D, _P, _P],
     "et_ufea_integrals_f64": [ctypes.POINTER(UMesh), _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "et_csr_spmv_f64": [_I64, _P, _P, _P, _P, _P, _P],
+    "et_umesh_pattern": [ctypes.POINTER(UMesh), _P, _P, _I64, ctypes.POINTER(ctypes.c_int64), _P],
+    "et_uassemble_f64": [ctypes.POINTER(UMesh), _P, _P, _P, _P, _P, _P],
+    "et_csr_cg_f64": [_I64, _P, _P, _P, _P, _P, _D, _I, _P, _I64, ctypes.POINTER(ctypes.c_int),
+                      ctypes.POINTER(ctypes.c_double), _P],
+    "et_csr_gmres_f64": [_I64, _P, _P, _P, _P, _P, _D, _I, _I, _P, _I64, ctypes.POINTER(ctypes.c_int),
+                         ctypes.POINTER(ctypes.c_double), _P],
     "et_ucg_f64": [ctypes.POINTER(UMesh), _P, _P, _P, _P, _D, _I, _P, _I64, ctypes.POINTER(ctypes.c_int),
                    ctypes.POINTER(ctypes.c_double), _P],
     "et_ugmres_f64": [ctypes.POINTER(UMesh), _P, _P, _P, _P, _D, _I, _I, _P, _I64, ctypes.POINTER(ctypes.c_int),

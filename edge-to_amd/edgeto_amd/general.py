@@ -13,13 +13,16 @@ on the device (et_umesh_star); every operator runs in the library
 et_ufea_integrals_f64), and the filter is a CSR product (et_csr_spmv_f64) with
 its transpose stored as a second CSR matrix.
 
-Deviations: redundant nodes are removed and the element lists renumbered
+construct() assembles the operator as a CSR matrix on the device (pattern once,
+values per call: et_umesh_pattern / et_uassemble_f64); CuCG / CuGMRES use it by
+default for these kernels, as the reference does (matrix_free=False).
+
+Deviation: redundant nodes are removed and the element lists renumbered
 consistently (the reference renumbers elements_flat only and then evaluates the
-element matrices with the old numbering, geom/_mesh.py:172-190); construct()
-(an assembled CSR matrix for direct solvers) is not provided, the Krylov solvers
-apply the operator matrix-free.
+element matrices with the old numbering, geom/_mesh.py:172-190).
 """
 import ctypes
+import warnings
 
 import numpy as np
 import torch
@@ -251,9 +254,34 @@ class _UnstructuredKernel(StiffnessKernel):
                   _lib.f64(xt, self.n_el), float(penal), float(void), _lib.f64(out, self.n_el), _lib.stream())
         return out
 
+    def _pattern(self):
+        """CSR row offsets and columns of the assembled operator (built once, et_umesh_pattern)."""
+        if getattr(self, "_csr", None) is None:
+            dev = self.constraints.device
+            rp = torch.empty(self.shape[0] + 1, dtype=torch.int32, device=dev)
+            nnz = ctypes.c_int64(0)
+            _lib.call("et_umesh_pattern", ctypes.byref(self._um), _lib.i32(rp), None, 0, ctypes.byref(nnz),
+                      _lib.stream())
+            cols = torch.empty(int(nnz.value), dtype=torch.int32, device=dev)
+            _lib.call("et_umesh_pattern", ctypes.byref(self._um), _lib.i32(rp), _lib.i32(cols), int(nnz.value),
+                      ctypes.byref(nnz), _lib.stream())
+            self._csr = (rp, cols)
+        return self._csr
+
     def construct(self, rho):
-        raise NotImplementedError("assembled matrices (direct solvers) are not provided; "
-                                  "CuCG / CuGMRES apply the operator matrix-free")
+        """The assembled operator A_hat = M K(rho) M + I_c as a device CSR matrix
+        (kernels/_cuda.py:407-420); the sparsity pattern is built on the first call."""
+        rp, cols = self._pattern()
+        rho = rho.reshape(-1).contiguous()
+        vals = torch.empty(cols.numel(), dtype=torch.float64, device=cols.device)
+        _lib.call("et_uassemble_f64", ctypes.byref(self._um), _lib.f64(rho, self.n_el), self._cons_ptr(),
+                  _lib.i32(rp), _lib.i32(cols), _lib.f64(vals), _lib.stream())
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", UserWarning)  # torch marks sparse CSR tensors as beta
+            self.CSR = torch.sparse_csr_tensor(rp, cols, vals, size=self.shape)
+        self._csr_vals = vals
+        self.has_been_constructed = True
+        return self.CSR
 
     def __matmul__(self, rhs):
         return self.dot(rhs)

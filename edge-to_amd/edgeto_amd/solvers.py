@@ -144,14 +144,20 @@ class _Krylov(Solver):
 
     def __init__(self, kernel, maxiter=1000, tol=1e-5, matrix_free=None):
         super().__init__()
-        if matrix_free is False:
-            logger.warning("matrix_free=False: structured kernels are always applied matrix-free "
-                           "(no assembled matrix is built); the result is the same operator.")
+        general = hasattr(kernel, "_um")
+        if general:
+            # general meshes: assembled CSR by default, as the reference (solvers/_gpu.py:18-27)
+            matrix_free = False if matrix_free is None else bool(matrix_free)
+        else:
+            if matrix_free is False:
+                logger.warning("matrix_free=False: structured kernels are always applied matrix-free "
+                               "(no assembled matrix is built); the result is the same operator.")
+            matrix_free = True
         self.kernel = kernel
         self.last_x0 = None
         self.tol = tol
         self.maxiter = maxiter
-        self.matrix_free = True
+        self.matrix_free = matrix_free
         self.last_iterations = 0
         self.last_timings = (0.0, 0.0)
         self._work = None
@@ -165,7 +171,17 @@ class _Krylov(Solver):
         it, res = ctypes.c_int(0), ctypes.c_double(0.0)
         extra = (int(self._restart),) if self._restart else ()
         t0 = time.perf_counter()
-        if hasattr(k, "_um"):
+        if hasattr(k, "_um") and not self.matrix_free:
+            # general meshes, assembled: the operator of this density as CSR, then the iterations
+            A = k.construct(k.rho)
+            rp, cols = k._pattern()
+            status = getattr(_lib.lib(), self._cname)(
+                n, _lib.i32(rp), _lib.i32(cols), _lib.f64(k._csr_vals), _lib.f64(rhs, n), _lib.f64(x, n),
+                float(self.tol), *extra, int(self.maxiter), _lib.f64(self._work), self._work.numel() * 8,
+                ctypes.byref(it), ctypes.byref(res), _lib.stream())
+            name = self._cname
+            del A
+        elif hasattr(k, "_um"):
             # general meshes: the same iterations on the unstructured operator
             status = getattr(_lib.lib(), self._uname)(
                 ctypes.byref(k._um), _lib.f64(k.rho, k.n_el), k._cons_ptr(), _lib.f64(rhs, n), _lib.f64(x, n),
@@ -209,6 +225,7 @@ class CG(_Krylov):
     """Conjugate gradients without preconditioning (solvers/_gpu.py:11-56)."""
     _name = "et_cg_f64"
     _uname = "et_ucg_f64"
+    _cname = "et_csr_cg_f64"
 
 
 class GMRES(_Krylov):
@@ -216,6 +233,7 @@ class GMRES(_Krylov):
     default restart length."""
     _name = "et_gmres_f64"
     _uname = "et_ugmres_f64"
+    _cname = "et_csr_gmres_f64"
     _restart = 20
 
 

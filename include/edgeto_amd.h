@@ -362,6 +362,26 @@ int et_ufea_integrals_f64(const et_umesh* mesh, int nb, const double* B, const i
  * transpose).  Each row sums in a fixed order. */
 int et_csr_spmv_f64(int64_t n_rows, const int* row_ptr, const int* cols, const double* vals,
                     const double* x, double* y, hipStream_t stream);
+/* The assembled operator as CSR, one row per DOF, int32 offsets and columns (the
+ * reference's construct(), kernels/_cuda.py:407-420 with matmat_*_wcon, :837-996):
+ * row (i, k) lists the DOFs (j, d) of every node j sharing an element with node i, node
+ * ids ascending.  et_umesh_pattern fills row_ptr (n_nodes * dof + 1) and *nnz, and cols
+ * when cap >= *nnz (call it once with cols = NULL to size cols); synchronises `stream`.
+ * et_uassemble_f64 writes vals (nnz) for densities E: A_hat = M K(E) M + I_c (constrained
+ * rows and columns zero, 1 on the constrained diagonal), summed in star order. */
+int et_umesh_pattern(const et_umesh* mesh, int* row_ptr, int* cols, int64_t cap, int64_t* nnz,
+                     hipStream_t stream);
+int et_uassemble_f64(const et_umesh* mesh, const double* E, const uint8_t* cons, const int* row_ptr,
+                     const int* cols, double* vals, hipStream_t stream);
+/* CG / GMRES(restart) on a CSR matrix (n_rows unknowns): the solvers of
+ * et_cg_f64 / et_gmres_f64 on an assembled operator (CuCG / CuGMRES with
+ * matrix_free=False, solvers/_gpu.py:40-52). */
+int et_csr_cg_f64(int64_t n_rows, const int* row_ptr, const int* cols, const double* vals,
+                  const double* b, double* x, double tol, int maxiter, void* work, int64_t work_bytes,
+                  int* iters, double* residual, hipStream_t stream);
+int et_csr_gmres_f64(int64_t n_rows, const int* row_ptr, const int* cols, const double* vals,
+                     const double* b, double* x, double tol, int restart, int maxiter, void* work,
+                     int64_t work_bytes, int* iters, double* residual, hipStream_t stream);
 /* CG / GMRES(restart) on the general-mesh operator: et_cg_f64 / et_gmres_f64 semantics
  * (CuCG / CuGMRES with a Uniform or General stiffness kernel, solvers/_gpu.py:11-107). */
 int et_ucg_f64(const et_umesh* mesh, const double* E, const uint8_t* cons, const double* b, double* x,

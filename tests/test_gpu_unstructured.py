@@ -185,10 +185,44 @@ def _clamped(nodes, dim):
     return face, cons
 
 
-@pytest.mark.parametrize("name,solver", [("tri", "CuCG"), ("tet", "CuCG"), ("tet", "CuGMRES")])
-def test_krylov_on_general_mesh(name, solver):
-    """CuCG / CuGMRES with a general stiffness kernel against a direct solve of the assembled
-    oracle operator."""
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_construct_assembled_operator(name):
+    """construct(): the device CSR equals the oracle's assembled A_hat (constraints included),
+    its columns are sorted per row, and its product equals the matrix-free operator."""
+    ET, L = _et()
+    nodes, els = MESHES[name]()
+    mesh = ET.CuGeneralMesh(nodes, els)
+    kernel = ET.CuGeneralStiffnessKernel(mesh)
+    dim = nodes.shape[1]
+    n = nodes.shape[0] * dim
+    rng = np.random.default_rng(11)
+    E = rng.uniform(0.05, 1.0, len(els))
+    cons = np.zeros(n, bool)
+    cons[rng.choice(n, n // 9, replace=False)] = True
+    kernel.set_rho(torch.tensor(E, device="cuda"))
+    kernel.add_constraints(np.nonzero(cons)[0])
+    A = kernel.construct(kernel.rho)
+    rp, cols = (t.cpu().numpy() for t in kernel._pattern())
+    vals = kernel._csr_vals.cpu().numpy()
+    import scipy.sparse as sps
+    Ad = sps.csr_matrix((vals, cols, rp), shape=(n, n)).toarray()
+    Ks = _oracle_mats(nodes, els)[0]
+    Ao = ou.assemble(Ks, els, E, cons, n, dim).toarray()
+    assert rel(Ad, Ao) < 1e-12
+    assert all(np.all(np.diff(cols[rp[r]:rp[r + 1]]) > 0) for r in range(n))
+    u = torch.tensor(rng.standard_normal(n), device="cuda")
+    y = torch.empty_like(u)
+    L.call("et_csr_spmv_f64", n, L.i32(kernel._pattern()[0]), L.i32(kernel._pattern()[1]), L.f64(kernel._csr_vals),
+           L.f64(u), L.f64(y), L.stream())
+    assert rel(y.cpu().numpy(), kernel.dot(u).cpu().numpy()) < 1e-12
+    assert A.layout == torch.sparse_csr
+
+
+@pytest.mark.parametrize("name,solver,mf", [("tri", "CuCG", None), ("tet", "CuCG", None), ("tet", "CuCG", True),
+                                            ("tet", "CuGMRES", None), ("tri", "CuGMRES", True)])
+def test_krylov_on_general_mesh(name, solver, mf):
+    """CuCG / CuGMRES with a general stiffness kernel (assembled CSR by default, or matrix-free)
+    against a direct solve of the assembled oracle operator."""
     ET, L = _et()
     nodes, els = MESHES[name]()
     dim = nodes.shape[1]
@@ -203,7 +237,8 @@ def test_krylov_on_general_mesh(name, solver):
     Ks = _oracle_mats(nodes, els)[0]
     A = ou.assemble(Ks, els, E, cons, b.size, dim).tocsc()
     xd = spla.spsolve(A, b)
-    s = getattr(ET, solver)(kernel, maxiter=5000, tol=1e-11)
+    s = getattr(ET, solver)(kernel, maxiter=5000, tol=1e-11, matrix_free=mf)
+    assert s.matrix_free == bool(mf)
     x, res = s.solve(torch.tensor(b, device="cuda"), use_last=False)
     assert res < 1e-10
     assert rel(x.cpu().numpy(), xd) < 1e-8
