@@ -35,6 +35,7 @@ from .general import (CuGeneralFilter, CuGeneralMesh, CuGeneralStiffnessKernel, 
                       GeneralFilter, GeneralMesh, GeneralStiffnessKernel, UniformStiffnessKernel)
 from .physics import (AutoStiffness, element_matrices, tetrahedron_element_stiffness,
                       triangle_element_stiffness)
+from . import torch_ops
 
 _lib.load(require_gpu=False)
 

@@ -62,3 +62,22 @@ def test_product_filter_offsets_match_oracle():
         b = filters.offsets_weights(rmin, sp)
         assert np.array_equal(a[0], b[0]) and np.allclose(a[1], b[1], rtol=0, atol=1e-16)
 
+
+
+def test_torch_ops_registered_with_shape_functions():
+    """The hot-path ops are torch custom operators; their shape functions run under fake
+    tensors (no device work), as tracing needs."""
+    import torch
+    import edgeto_amd  # noqa: F401 (registers the ops)
+    from torch._subclasses.fake_tensor import FakeTensorMode
+    for name in ("matvec", "filter_fwd", "filter_adj", "simp_adjoint", "oc_update"):
+        assert hasattr(torch.ops.edgeto_amd, name)
+    with FakeTensorMode():
+        u = torch.empty(81, dtype=torch.float64)
+        K0 = torch.empty(576, dtype=torch.float64)
+        E = torch.empty(8, dtype=torch.float64)
+        y = torch.ops.edgeto_amd.matvec(K0, E, None, u, [2, 2, 2])
+        assert y.shape == u.shape and y.dtype == torch.float64
+        x = torch.empty(8, dtype=torch.float64)
+        z = torch.ops.edgeto_amd.oc_update(x, x, 1.0, 8.0, 0.3, 0.2)
+        assert z.shape == x.shape
