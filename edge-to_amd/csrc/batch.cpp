@@ -11,11 +11,15 @@
 // check) overlap with the other problems' work.
 //
 // Per OC iteration a problem issues one captured graph and one solve.  The
-// graph holds everything between two solves, captured once per problem and
-// replayed: compliance f.u, the fused SIMP adjoint with the filter transpose,
-// the OC update (device bisection, no host round trip), the design change, the
-// filter and the SIMP map of the new design.  The solve itself (hierarchy
-// rebuild and PCG with its data-dependent iteration count) is issued directly.
+// graph holds everything between two solves: the fused SIMP adjoint with the
+// filter transpose, the OC update (device bisection, no host round trip), the
+// design change, the filter and the SIMP map of the new design.  It depends on
+// the two penalties involved (the previous solve's and the next one's), so a
+// problem keeps one graph per pair that occurs -- one without a penalty
+// continuation, one per step of it otherwise -- captured on first use and
+// replayed.  The solve itself (hierarchy rebuild and PCG with its
+// data-dependent iteration count) and the compliance f.u after it are issued
+// directly; the compliance feeds the stopping test.
 //
 // The sequence is CuTopOpt.optimize (TopOpt/_cuda.py:304-382) per problem:
 // x = clip(x0, void, 1); initial solve; then per iteration update_design,
@@ -42,7 +46,7 @@ struct BatchProblem {
   Grid g{};
   int dof = 0, k0_flags = 0;
   int64_t n_dof = 0, n_el = 0;
-  std::vector<double> K0h, fh, x0h, wh;
+  std::vector<double> K0h, fh, x0h, wh, sched;
   std::vector<uint8_t> consh;
   std::vector<int32_t> offh;
   hipStream_t stream = nullptr;
@@ -53,11 +57,18 @@ struct BatchProblem {
   uint8_t* cons = nullptr;
   int32_t* off = nullptr;
   void* ocw = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // captured between-solve graphs, keyed by (adjoint penalty, forward penalty)
+  struct Captured {
+    double pa, pf;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<Captured> graphs;
   // results
   std::vector<double> comp, iter_ms, change;
   std::vector<int> pcg;
+  int done = 0;
+  bool converged = false;
   int status = ET_OK;
   std::string error;
 };
@@ -72,10 +83,11 @@ struct BatchEngine {
 
 void BatchEngine::release() {
   for (auto& p : P) {
-    if (p.exec) (void)hipGraphExecDestroy(p.exec);
-    if (p.graph) (void)hipGraphDestroy(p.graph);
-    p.exec = nullptr;
-    p.graph = nullptr;
+    for (auto& c : p.graphs) {
+      if (c.exec) (void)hipGraphExecDestroy(c.exec);
+      if (c.graph) (void)hipGraphDestroy(c.graph);
+    }
+    p.graphs.clear();
     if (p.mg_ok) p.mg.destroy();
     p.mg_ok = false;
     for (double** v : {&p.K0, &p.f, &p.U, &p.x, &p.xn, &p.xt, &p.E, &p.dc, &p.w, &p.norm, &p.scal, &p.partial})
@@ -112,6 +124,10 @@ int setup_problem(BatchProblem& p, int precision) {
   p.x0h.assign(d.x0, d.x0 + p.n_el);
   p.offh.assign(d.offsets, d.offsets + (size_t)d.n_off * d.dim);
   p.wh.assign(d.weights, d.weights + d.n_off);
+  ET_REQUIRE(d.n_schedule >= 0 && (d.n_schedule == 0 || d.penal_schedule != nullptr),
+             "et_batch_create: penalty schedule without values");
+  p.sched.assign(d.penal_schedule, d.penal_schedule + d.n_schedule);
+  for (const double v : p.sched) ET_REQUIRE(v > 0.0 && std::isfinite(v), "et_batch_create: bad penalty in the schedule");
   ET_CHECK_HIP(hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking));
   const size_t vb = p.n_dof * sizeof(double), eb = p.n_el * sizeof(double);
   ET_CHECK_HIP(hipMalloc(&p.K0, nd * nd * sizeof(double)));
@@ -152,19 +168,62 @@ int setup_problem(BatchProblem& p, int precision) {
   return ET_OK;
 }
 
-// the work between two solves, in stream order (captured into the problem's graph):
-// c = f.u, dc = H^T s, x_new = OC(x, dc), |x_new - x|^2, x = x_new, xt = H x, E = SIMP(xt)
-int between_solves(BatchProblem& p) {
+// penalty of the densities of OC iteration k (k = 0 also for the initial solve)
+double penalty_at(const BatchProblem& p, int k) {
+  if (p.sched.empty()) return p.d.penal;
+  return p.sched[std::min<size_t>((size_t)k, p.sched.size() - 1)];
+}
+
+// the work between two solves, in stream order (captured into a graph): dc = H^T s
+// with the previous solve's penalty pa, x_new = OC(x, dc), |x_new - x|^2, x = x_new,
+// xt = H x, E = SIMP(xt) with the next solve's penalty pf
+int between_solves(BatchProblem& p, double pa, double pf) {
   const et_batch_problem& d = p.d;
   hipStream_t s = p.stream;
-  ET_TRY(dot(p.n_dof, p.f, p.U, p.partial, p.scal, s));
-  ET_TRY(simp_adjoint(p.g, p.K0, p.U, p.xt, d.penal, d.void_, d.n_off, p.off, p.w, p.norm, p.k0_flags, p.dc, s));
+  ET_TRY(simp_adjoint(p.g, p.K0, p.U, p.xt, pa, d.void_, d.n_off, p.off, p.w, p.norm, p.k0_flags, p.dc, s));
   ET_TRY(oc_update(p.n_el, p.x, p.dc, d.A, d.A * p.n_el, d.volfrac, d.move, p.xn, p.ocw, nullptr, s));
   ET_TRY(sqdist(p.n_el, p.xn, p.x, p.partial, p.scal + 1, s));
   copy_kernel<<<grid_for(p.n_el), kBlock, 0, s>>>(p.n_el, p.xn, p.x);
   ET_CHECK_LAUNCH();
   ET_TRY(filter_fwd(p.g, d.n_off, p.off, p.w, p.x, p.xt, s));
-  ET_TRY(simp_forward(p.n_el, p.xt, d.penal, d.void_, 0, p.E, s));
+  ET_TRY(simp_forward(p.n_el, p.xt, pf, d.void_, 0, p.E, s));
+  return ET_OK;
+}
+
+// the executable graph for the penalty pair (pa, pf), captured on first use
+int graph_for(BatchProblem& p, double pa, double pf, hipGraphExec_t* out) {
+  for (const auto& c : p.graphs)
+    if (c.pa == pa && c.pf == pf) {
+      *out = c.exec;
+      return ET_OK;
+    }
+  hipStream_t s = p.stream;
+  ET_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int st = between_solves(p, pa, pf);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (st != ET_OK || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    ET_TRY(st);
+    ET_CHECK_HIP(e);
+  }
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    ET_CHECK_HIP(ei);
+  }
+  p.graphs.push_back({pa, pf, g, x});
+  *out = x;
+  return ET_OK;
+}
+
+// compliance f.u of the last solve (synchronises the problem's stream)
+int compliance(BatchProblem& p, double* c) {
+  hipStream_t s = p.stream;
+  ET_TRY(dot(p.n_dof, p.f, p.U, p.partial, p.scal, s));
+  ET_CHECK_HIP(hipMemcpyAsync(c, p.scal, sizeof(double), hipMemcpyDeviceToHost, s));
+  ET_CHECK_HIP(hipStreamSynchronize(s));
   return ET_OK;
 }
 
@@ -184,44 +243,46 @@ int run_problem(BatchProblem& p, int n_iter) {
   const int nt[2] = {d.tile_threads_f32, d.tile_threads_f64};
   const int ns[2] = {d.tile_slices_f32, d.tile_slices_f64};
   set_thread_wht_tiles(nt, ns);
-  p.comp.assign(1, 0.0);
+  p.comp.clear();
   p.pcg.clear();
   p.iter_ms.clear();
   p.change.clear();
+  p.done = 0;
+  p.converged = false;
   ET_CHECK_HIP(hipMemcpyAsync(p.xn, p.x0h.data(), p.n_el * sizeof(double), hipMemcpyHostToDevice, s));
   // x = clip(x0, void, 1); xt = H x; E = SIMP(xt); U = 0
   ET_TRY(simp_forward(p.n_el, p.xn, 1.0, d.void_, 1, p.x, s));
   ET_TRY(filter_fwd(p.g, d.n_off, p.off, p.w, p.x, p.xt, s));
-  ET_TRY(simp_forward(p.n_el, p.xt, d.penal, d.void_, 0, p.E, s));
+  ET_TRY(simp_forward(p.n_el, p.xt, penalty_at(p, 0), d.void_, 0, p.E, s));
   ET_CHECK_HIP(hipMemsetAsync(p.U, 0, p.n_dof * sizeof(double), s));
   ET_TRY(solve(p));
-  if (p.exec == nullptr && n_iter > 0) {
-    ET_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const int st = between_solves(p);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(s, &g);
-    ET_TRY(st);
-    ET_CHECK_HIP(e);
-    p.graph = g;
-    ET_CHECK_HIP(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0));
-  }
-  double h[2];
+  double c = 0.0, h = 0.0;
+  ET_TRY(compliance(p, &c));
+  p.comp.push_back(c);
   for (int k = 0; k < n_iter; ++k) {
     const auto t0 = clock::now();
-    ET_CHECK_HIP(hipGraphLaunch(p.exec, s));
-    ET_CHECK_HIP(hipMemcpyAsync(h, p.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    // iteration k's update uses the sensitivities of the previous solve (penalty of
+    // iteration k-1, or of the initial solve) and builds the densities with iteration k's
+    const double pf = penalty_at(p, k);
+    hipGraphExec_t exec = nullptr;
+    ET_TRY(graph_for(p, penalty_at(p, k > 0 ? k - 1 : 0), pf, &exec));
+    ET_CHECK_HIP(hipGraphLaunch(exec, s));
+    ET_CHECK_HIP(hipMemcpyAsync(&h, p.scal + 1, sizeof(double), hipMemcpyDeviceToHost, s));
     ET_CHECK_HIP(hipStreamSynchronize(s));
-    p.comp[k] = h[0];
-    p.change.push_back(std::sqrt(h[1] / (double)p.n_el));
+    const double change = std::sqrt(h / (double)p.n_el);
+    p.change.push_back(change);
     ET_TRY(solve(p));
-    p.comp.push_back(0.0);
+    const double c_old = c;
+    ET_TRY(compliance(p, &c));
+    p.comp.push_back(c);
     p.iter_ms.push_back(std::chrono::duration<double, std::milli>(clock::now() - t0).count());
+    p.done = k + 1;
+    // CuTopOpt.optimize's test (TopOpt/_cuda.py:366-373); never true with zero tolerances
+    if (change < d.ch_tol && std::fabs(c - c_old) / c < d.fun_tol && pf == d.penal) {
+      p.converged = true;
+      break;
+    }
   }
-  // compliance of the last solve
-  ET_TRY(dot(p.n_dof, p.f, p.U, p.partial, p.scal, s));
-  ET_CHECK_HIP(hipMemcpyAsync(h, p.scal, sizeof(double), hipMemcpyDeviceToHost, s));
-  ET_CHECK_HIP(hipStreamSynchronize(s));
-  p.comp.back() = h[0];
   const int zero[2] = {0, 0};
   set_thread_wht_tiles(zero, zero);
   return ET_OK;
@@ -282,6 +343,13 @@ int batch_run(BatchEngine* b, int n_iter, int n_workers) {
   return ET_OK;
 }
 
+int batch_status(BatchEngine* b, int i, int* n_done, int* converged) {
+  ET_REQUIRE(b && i >= 0 && i < (int)b->P.size(), "et_batch_status: bad problem index");
+  if (n_done) *n_done = b->P[i].done;
+  if (converged) *converged = b->P[i].converged ? 1 : 0;
+  return ET_OK;
+}
+
 int batch_result(BatchEngine* b, int i, double* comp, int* pcg, double* iter_ms, double* x_host) {
   ET_REQUIRE(b && i >= 0 && i < (int)b->P.size(), "et_batch_result: bad problem index");
   BatchProblem& p = b->P[i];
@@ -336,6 +404,16 @@ int et_batch_result(et_batch* handle, int i, double* comp, int* pcg, double* ite
     return et::batch_result(handle->impl, i, comp, pcg, iter_ms, x_host);
   } catch (...) {
     et::set_error("et_batch_result: host exception");
+    return ET_EINVAL;
+  }
+}
+
+int et_batch_status(et_batch* handle, int i, int* n_iterations, int* converged) {
+  try {
+    ET_REQUIRE(handle, "et_batch_status: null handle");
+    return et::batch_status(handle->impl, i, n_iterations, converged);
+  } catch (...) {
+    et::set_error("et_batch_status: host exception");
     return ET_EINVAL;
   }
 }

@@ -16,6 +16,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .MaterialModels import PenalizedMultiMaterial
 
 logger = logging.getLogger(__name__)
 
@@ -82,12 +83,19 @@ class TopOpt:
         Fs = np.asarray(Fs, dtype=np.float64)
         if Fs.shape[0] != idx.shape[0] and Fs.shape[0] > 1:
             raise ValueError("Fs shape does not match positions shape and is not broadcastable.")
-        idx = np.asarray(idx).reshape(-1)
+        self._set_loads(np.asarray(idx).reshape(-1), Fs)
+
+    def _set_loads(self, ids, Fs):
+        """f[node * dof + i] = Fs[., i]; a node listed more than once keeps its last row (NumPy
+        assignment order, made explicit: a device scatter with repeated indices has none)."""
         dof = self.mesh.dof
         for i in range(dof):
-            vals = Fs[0, i] if Fs.shape[0] == 1 else Fs[:, i]
-            self.f[torch.as_tensor(idx * dof + i, device=self.f.device)] = torch.as_tensor(
-                vals, dtype=torch.float64, device=self.f.device)
+            dofs = ids * dof + i
+            vals = np.broadcast_to(Fs[0, i] if Fs.shape[0] == 1 else Fs[:, i], dofs.shape)
+            _, first_rev = np.unique(dofs[::-1], return_index=True)
+            keep = len(dofs) - 1 - first_rev
+            self.f[torch.as_tensor(dofs[keep], device=self.f.device)] = torch.as_tensor(
+                np.ascontiguousarray(vals[keep]), dtype=torch.float64, device=self.f.device)
 
     def add_BC_nodal(self, node_ids, BCs):
         node_ids = np.asarray(node_ids).reshape(-1)
@@ -107,12 +115,7 @@ class TopOpt:
         Fs = np.asarray(Fs, dtype=np.float64)
         if Fs.shape[0] != node_ids.shape[0] and Fs.shape[0] > 1:
             raise ValueError("Fs shape does not match positions shape and is not broadcastable.")
-        dof = self.mesh.dof
-        ids = node_ids.reshape(-1)
-        for i in range(dof):
-            vals = Fs[0, i] if Fs.shape[0] == 1 else Fs[:, i]
-            self.f[torch.as_tensor(ids * dof + i, device=self.f.device)] = torch.as_tensor(
-                vals, dtype=torch.float64, device=self.f.device)
+        self._set_loads(node_ids.reshape(-1), Fs)
 
     # ------------------------------------------------------- post-processing
     def FEA_integrals(self, rho):
@@ -157,6 +160,12 @@ class TopOpt:
         return strain, stress, energy, vm
 
     # --------------------------------------------------------------- OC / loop
+    def _multi(self):
+        """Multi-material updates (one OC multiplier per material): any PenalizedMultiMaterial,
+        one material included, as the reference's class methods do for m = 1."""
+        mm = self.material_model
+        return mm.base_properties()["n_sets"] > 1 or isinstance(mm, PenalizedMultiMaterial)
+
     def _update_design_multi(self, rho, df):
         """One multiplier per material (n_sets = m), TopOpt/_cuda.py:242-280."""
         mm = self.material_model
@@ -183,7 +192,7 @@ class TopOpt:
         return out
 
     def update_design(self, rho, df):
-        if self.material_model.base_properties()["n_sets"] > 1:
+        if self._multi():
             return self._update_design_multi(rho, df)
         rho = rho.reshape(-1).contiguous()
         df = df.reshape(-1).contiguous()
@@ -205,7 +214,7 @@ class TopOpt:
     @property
     def last_oc_steps(self):
         """Bisection steps of the last update_design (reads the device state: synchronises)."""
-        if self.material_model.base_properties()["n_sets"] > 1:
+        if self._multi():
             return getattr(self, "_multi_steps", 0)
         return int(self._oc_work[:5].view(torch.int32)[9].item())
 
@@ -237,7 +246,7 @@ class TopOpt:
 
     def solve_and_process(self, rho, iteration):
         mm = self.material_model
-        if mm.base_properties()["n_sets"] > 1:
+        if self._multi():
             return self._solve_and_process_multi(rho, iteration)
         rho_ = self.filter_kernel.dot(rho)
         rho__ = mm(rho_, iteration)
@@ -255,7 +264,7 @@ class TopOpt:
     def optimize(self, save_comp_history=False, save_change_history=False, save_rho_history=False,
                  rho_init=None):
         flag = False
-        multi = self.material_model.base_properties()["n_sets"] > 1
+        multi = self._multi()
         if rho_init is None:
             rho = self.material_model.init_desvars(_n_el(self.kernel))
         elif multi:

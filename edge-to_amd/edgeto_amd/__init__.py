@@ -36,6 +36,7 @@ from .general import (CuGeneralFilter, CuGeneralMesh, CuGeneralStiffnessKernel, 
 from .physics import (AutoStiffness, element_matrices, tetrahedron_element_stiffness,
                       triangle_element_stiffness)
 from . import torch_ops
+from . import randomizer  # noqa: F401  (generators; ConfigSolve is imported on use)
 
 _lib.load(require_gpu=False)
 

@@ -94,6 +94,7 @@ _SIGS = {
     "et_batch_create": [ctypes.POINTER(_P), _I, _P, _I],
     "et_batch_run": [_P, _I, _I, ctypes.POINTER(ctypes.c_double)],
     "et_batch_result": [_P, _I, _P, _P, _P, _P],
+    "et_batch_status": [_P, _I, _P, _P],
     "et_batch_destroy": [_P],
     "et_oc_update_elem_f64": [_I64, _P, _P, _P, _D, _D, _D, _P, _P, ctypes.POINTER(ctypes.c_int), _P],
     "et_umesh_star": [_I64, _I64, _P, _P, _P, _P, _P, _P],

@@ -39,7 +39,8 @@ class BatchProblemC(ctypes.Structure):
                 ("tol", ctypes.c_double), ("maxiter", ctypes.c_int), ("n_level", ctypes.c_int),
                 ("n_smooth", ctypes.c_int), ("tile_threads_f32", ctypes.c_int),
                 ("tile_threads_f64", ctypes.c_int), ("tile_slices_f32", ctypes.c_int),
-                ("tile_slices_f64", ctypes.c_int)]
+                ("tile_slices_f64", ctypes.c_int), ("penal_schedule", ctypes.c_void_p),
+                ("n_schedule", ctypes.c_int), ("ch_tol", ctypes.c_double), ("fun_tol", ctypes.c_double)]
 
 
 def gmg_levels(nel, coarse_dof=2000, max_coarse_dof=6000, max_levels=6):
@@ -78,9 +79,16 @@ def load_tiles(path=TILE_TABLE):
 
 
 class BatchTopOpt:
-    """problems: list of dicts with keys dim, nel, clamp (node ids, all DOFs fixed),
-    loads (node ids), dirs (load vectors, one row per load), vf, rmin, and optionally
-    penal (3), move (0.2), x0 (scalar or array, default vf), n_level, nu (0.3)."""
+    """problems: list of dicts with keys dim, nel, loads (node ids), dirs (load vectors, one
+    row per load; a node listed twice keeps its last vector), vf, rmin, the supports as
+    clamp (node ids, every DOF fixed) and / or fix_nodes with fix_flags (node ids and their
+    per-axis 0/1 flags, one row per node or one row for all), and optionally penal (3),
+    penal_schedule (penalty per OC iteration, the last one repeating; the final value
+    should be penal, which the stopping test waits for), ch_tol and fun_tol (stop once the
+    RMS design change and the relative compliance change are below both, at the final
+    penalty; default 0: run every iteration), move (0.2), x0 (scalar or array, default vf),
+    n_level, nu (0.3), h (element edge length, one value or one per axis; default 1: the
+    element matrix, the element measure and the filter's spacing follow it)."""
 
     def __init__(self, problems, precision="mixed", tol=1e-8, maxiter=1000, n_smooth=2, void=1e-9,
                  tiles=None):
@@ -96,23 +104,31 @@ class BatchTopOpt:
             corners = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], dtype=np.float64) if dim == 2 else \
                 np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]],
                          dtype=np.float64)
-            K, _, _, A = auto_stiffness(corners, 1.0, float(pb.get("nu", 0.3)))
+            h = np.broadcast_to(np.asarray(pb.get("h", 1.0), dtype=np.float64), (dim,))
+            K, _, _, A = auto_stiffness(corners * h[None, :], 1.0, float(pb.get("nu", 0.3)))
             n_el = int(np.prod(nel))
             n_nodes = int(np.prod([n + 1 for n in nel]))
             n_dof = n_nodes * dim
             cons = np.zeros(n_dof, dtype=np.uint8)
-            clamp = np.asarray(pb["clamp"], dtype=np.int64)
+            clamp = np.asarray(pb.get("clamp", []), dtype=np.int64).reshape(-1)
             for d in range(dim):
                 cons[clamp * dim + d] = 1
+            if "fix_nodes" in pb:
+                ids = np.asarray(pb["fix_nodes"], dtype=np.int64).reshape(-1)
+                flags = np.asarray(pb["fix_flags"]).astype(bool).reshape(-1, dim)
+                flags = np.broadcast_to(flags, (len(ids), dim))
+                for d in range(dim):
+                    cons[ids[flags[:, d]] * dim + d] = 1
             f = np.zeros(n_dof)
             for node, vec in zip(np.asarray(pb["loads"], dtype=np.int64), np.asarray(pb["dirs"], dtype=np.float64)):
                 f[node * dim:(node + 1) * dim] = vec
             x0 = np.broadcast_to(np.asarray(pb.get("x0", pb["vf"]), dtype=np.float64), (n_el,)).copy()
-            offs, w = _offsets_weights(float(pb["rmin"]), [1.0] * dim)
+            offs, w = _offsets_weights(float(pb["rmin"]), [float(v) for v in h])
             K = np.ascontiguousarray(K, dtype=np.float64)
             offs = np.ascontiguousarray(offs, dtype=np.int32)
             w = np.ascontiguousarray(w, dtype=np.float64)
-            self._keep += [K, cons, f, x0, offs, w]
+            sched = np.ascontiguousarray(np.asarray(pb.get("penal_schedule", []), dtype=np.float64).reshape(-1))
+            self._keep += [K, cons, f, x0, offs, w, sched]
             n_level = int(pb.get("n_level") or gmg_levels(nel))
             t = self.tiles.get(size_class(dim, nel), {})
             c = arr[i]
@@ -128,6 +144,9 @@ class BatchTopOpt:
             c.tile_threads_f64 = int(t.get("threads_f64", 0))
             c.tile_slices_f32 = int(t.get("slices_f32", 0))
             c.tile_slices_f64 = int(t.get("slices_f64", 0))
+            c.penal_schedule = sched.ctypes.data if sched.size else None
+            c.n_schedule = int(sched.size)
+            c.ch_tol, c.fun_tol = float(pb.get("ch_tol", 0.0)), float(pb.get("fun_tol", 0.0))
             self.n_el.append(n_el)
             self.n_dof.append(n_dof)
             self.levels.append(n_level)
@@ -144,9 +163,10 @@ class BatchTopOpt:
             self._handle = None
 
     def run(self, n_iter, n_workers=None):
-        """Initial solve plus n_iter OC iterations of every problem.  Returns a dict with the
-        wall time and, per problem, the compliance history, PCG iteration counts and the
-        wall time of each OC iteration."""
+        """Initial solve plus up to n_iter OC iterations of every problem.  Returns a dict with
+        the wall time and, per problem, the OC iterations run, whether the stopping test
+        ended them, the compliance history (initial solve first), PCG iteration counts and
+        the wall time of each OC iteration."""
         if n_workers is None:
             n_workers = int(self.tiles.get("workers", 8))
         wall = ctypes.c_double(0.0)
@@ -154,12 +174,16 @@ class BatchTopOpt:
         self.last_iters = int(n_iter)
         out = {"wall_ms": wall.value, "n_workers": int(n_workers), "problems": []}
         for i in range(len(self.problems)):
+            done, conv = ctypes.c_int(0), ctypes.c_int(0)
+            _lib.call("et_batch_status", self._handle, i, ctypes.byref(done), ctypes.byref(conv))
+            k = done.value
             comp = np.zeros(n_iter + 1)
             pcg = np.zeros(n_iter + 1, dtype=np.int32)
             ms = np.zeros(max(n_iter, 1))
             _lib.call("et_batch_result", self._handle, i, comp.ctypes.data_as(ctypes.c_void_p),
                       pcg.ctypes.data_as(ctypes.c_void_p), ms.ctypes.data_as(ctypes.c_void_p), None)
-            out["problems"].append({"compliance": comp, "pcg": pcg, "iter_ms": ms[:n_iter],
+            out["problems"].append({"compliance": comp[:k + 1], "pcg": pcg[:k + 1], "iter_ms": ms[:k],
+                                    "iterations": k, "converged": bool(conv.value),
                                     "n_level": self.levels[i]})
         return out
 

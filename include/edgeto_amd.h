@@ -506,8 +506,8 @@ int et_mg_profile_read(et_mg* handle, double* ms, int64_t* launches);
  * GMG-PCG solver resident; et_batch_run runs, for every problem, the initial solve and
  * n_iter OC iterations, n_workers host threads each driving one problem at a time on
  * that problem's stream (largest problems first).  Between two solves a problem issues
- * one captured graph (compliance, fused SIMP adjoint, OC update, design change, filter,
- * SIMP map), captured on its first iteration and replayed.  Synchronous.  No
+ * one captured graph (fused SIMP adjoint, OC update, design change, filter, SIMP map),
+ * captured the first time its pair of penalties occurs and replayed.  Synchronous.  No
  * counterpart in the reference (it optimises one problem per process). */
 typedef struct et_batch et_batch;
 typedef struct et_batch_problem {
@@ -525,15 +525,28 @@ typedef struct et_batch_problem {
   int maxiter, n_level, n_smooth;
   /* transformed-basis tiles of this problem's sweeps (0 = library default) */
   int tile_threads_f32, tile_threads_f64, tile_slices_f32, tile_slices_f64;
+  /* penalty continuation (NULL / 0: penal throughout): the densities of OC iteration k
+   * use penal_schedule[min(k, n_schedule - 1)], those of the initial solve entry 0 --
+   * the material model's penalty_schedule(p, k) evaluated by the caller
+   * (MaterialModels.py:47-60) */
+  const double* penal_schedule;
+  int n_schedule;
+  /* stop the problem after OC iteration k once the RMS design change is below ch_tol,
+   * the relative compliance change below fun_tol and iteration k's penalty equals penal
+   * (CuTopOpt.optimize, TopOpt/_cuda.py:366-373; zero tolerances: all n_iter iterations) */
+  double ch_tol, fun_tol;
 } et_batch_problem;
 int et_batch_create(et_batch** handle, int n, const et_batch_problem* problems, int precision);
 /* *wall_ms: host wall time of the whole run. */
 int et_batch_run(et_batch* handle, int n_iter, int n_workers, double* wall_ms);
-/* Results of problem i of the last run: compliance (n_iter + 1), PCG iterations
- * (n_iter + 1), wall time of each OC iteration in ms (n_iter), final design (n_elem,
- * host).  Any pointer may be NULL. */
+/* Results of problem i of the last run, for the k OC iterations it ran (k = n_iter
+ * unless it stopped early): compliance (k + 1), PCG iterations (k + 1), wall time of
+ * each OC iteration in ms (k), final design (n_elem, host).  Any pointer may be NULL. */
 int et_batch_result(et_batch* handle, int i, double* compliance, int* pcg_iterations,
                     double* iteration_ms, double* x_host);
+/* OC iterations problem i ran in the last run and whether it stopped on the convergence
+ * test (the flag that CuTopOpt.optimize returns).  Any pointer may be NULL. */
+int et_batch_status(et_batch* handle, int i, int* n_iterations, int* converged);
 int et_batch_destroy(et_batch* handle);
 
 #ifdef __cplusplus

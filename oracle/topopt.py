@@ -109,10 +109,19 @@ def solve_and_process(prob, x, material, solver, use_last=True):
 
 
 def optimize(prob, material, solver, x0, max_iter, move=0.2, ch_tol=1e-4, fun_tol=1e-6,
-             callback=None):
-    """Returns (x, flag, history dict with compliance, change, residual, pcg iterations)."""
+             callback=None, schedule=None):
+    """Returns (x, flag, history dict with compliance, change, residual, pcg iterations).
+    schedule(p, i): penalty of iteration i (the initial solve uses i = 0) with the stopping
+    test waiting for schedule(p, i) == p (MaterialModels.py:47-60, :95-98)."""
+    p_final = material.penalty
+
+    def set_penalty(i):
+        if schedule is not None:
+            material.penalty = schedule(p_final, i)
+
     x = material.forward(np.asarray(x0, dtype=np.float64), plain=True)
     hist = {"comp": [], "change": [], "res": [], "pcg": []}
+    set_penalty(0)
     c, dr, res, _ = solve_and_process(prob, x, material, solver)
     hist["comp"].append(c)
     hist["res"].append(res)
@@ -121,6 +130,7 @@ def optimize(prob, material, solver, x0, max_iter, move=0.2, ch_tol=1e-4, fun_to
     for i in range(max_iter):
         x_old, c_old = x.copy(), c
         x, _ = oc_update(x, dr, prob.A, prob.volume, material.volume_fraction, move, material)
+        set_penalty(i)
         c, dr, res, _ = solve_and_process(prob, x, material, solver)
         change = np.sqrt(np.square(x - x_old).mean())
         change_f = abs(c - c_old) / c
@@ -130,9 +140,10 @@ def optimize(prob, material, solver, x0, max_iter, move=0.2, ch_tol=1e-4, fun_to
         hist["pcg"].append(solver.last_iters)
         if callback is not None:
             callback(i, x, c)
-        if change < ch_tol and change_f < fun_tol:
+        if change < ch_tol and change_f < fun_tol and material.penalty == p_final:
             flag = True
             break
+    material.penalty = p_final
     return x, flag, hist
 
 

@@ -76,3 +76,45 @@ def test_batch_matches_single_problem_runs():
     again = batch.run(n_iter, n_workers=2)
     for i in range(len(problems)):
         assert np.array_equal(again["problems"][i]["compliance"], out["problems"][i]["compliance"])
+
+
+def test_batch_penalty_schedule_and_stopping_rule():
+    """Per-problem penalty continuation and CuTopOpt.optimize's stopping test in the engine:
+    the same iterations, the same stop, compliance and designs as the single-problem runs
+    (penalty 1, 1.5, 2 for iterations 0-2, then 3; loose tolerances so problems stop early)."""
+    import edgeto_amd as ET
+    sched = [1.0, 1.5, 2.0, 3.0]
+    n_iter = 30
+    problems = []
+    for i in range(3):
+        pb = _small_problem(i)
+        pb.update(penal_schedule=sched, ch_tol=0.02, fun_tol=0.01)
+        problems.append(pb)
+    batch = ET.BatchTopOpt(problems, precision="mixed", tiles={})
+    out = batch.run(n_iter, n_workers=3)
+    stopped = 0
+    for i, pb in enumerate(problems):
+        dim, nel = pb["dim"], pb["nel"]
+        stiff = lambda x0: ET.auto_stiffness(x0, 1.0, 0.3)
+        if dim == 2:
+            mesh = ET.CuStructuredMesh2D(*nel, *nel, stiffness=stiff)
+            filt = ET.CuStructuredFilter2D(mesh, pb["rmin"])
+        else:
+            mesh = ET.CuStructuredMesh3D(*nel, *nel, stiffness=stiff)
+            filt = ET.CuStructuredFilter3D(mesh, pb["rmin"])
+        kernel = ET.CuStructuredStiffnessKernel(mesh)
+        mat = ET.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=pb["vf"],
+                                penalty_schedule=lambda p, k: sched[min(k, len(sched) - 1)])
+        solver = ET.CuMultiGrid(mesh, kernel, tol=1e-8, n_smooth=2, n_level=batch.levels[i], precision="mixed")
+        tp = ET.CuTopOpt(mesh, mat, kernel, solver, filt, max_iter=n_iter, ch_tol=0.02, fun_tol=0.01, verbose=False)
+        tp.add_BC_nodal(pb["clamp"], np.ones((1, dim), dtype=np.int64))
+        tp.add_F_nodal(pb["loads"], pb["dirs"])
+        x, flag, hist = tp.optimize(save_comp_history=True, rho_init=np.full(mesh.n_el, pb["vf"]))
+        comp = np.array([tp.iteration_log[0]["comp"]] + hist["comp_history"])
+        got = out["problems"][i]
+        print(f"[batch] problem {i} {nel}: {got['iterations']} iterations, converged {got['converged']}", flush=True)
+        assert got["iterations"] == len(hist["comp_history"]) and got["converged"] == flag
+        assert np.max(np.abs(got["compliance"] - comp) / np.abs(comp)) <= 1e-10
+        assert np.max(np.abs(batch.design(i) - x.cpu().numpy())) <= 1e-10
+        stopped += got["iterations"] < n_iter
+    assert stopped >= 1

@@ -99,3 +99,46 @@ def test_segment_and_surface_shapes():
     others = [a for a in range(3) if a != axis[0]]
     ext = [int(pts[:, a].max() - pts[:, a].min()) + 1 for a in others]
     assert len(pts) == ext[0] * ext[1]
+
+
+def test_continuation_values():
+    """Penalty continuation of ConfigSolve.py:14-16: 1 + (i // 15)(p - 1)/5 below 75, then 3."""
+    from edgeto_amd.randomizer.ConfigSolve import continuation
+    got = [continuation(3.0, i) for i in (0, 14, 15, 29, 30, 44, 45, 60, 74, 75, 200)]
+    assert got == [1.0, 1.0, 1.4, 1.4, 1.8, 1.8, 2.2, 2.6, 2.6, 3.0, 3.0]
+
+
+@pytest.mark.parametrize("dims", [(4, 3), (4, 3, 2), (1, 5, 3)])
+def test_node_ids_follow_the_mesh_order(dims):
+    """Integer grid coordinates -> node ids, against the oracle mesh's node coordinates (the
+    reference's node order); out-of-grid points snap to the nearest boundary node."""
+    from oracle import mesh as omesh
+    from edgeto_amd.randomizer.ConfigSolve import _node_ids
+    d = np.array(dims)
+    nodes = omesh.nodes_2d(*dims, *dims) if len(dims) == 2 else omesh.nodes_3d(*dims, *dims)
+    assert np.array_equal(_node_ids(nodes, d), np.arange(len(nodes)))
+    outside = np.array([[-3] + [0] * (len(dims) - 1), list(d + 2)])
+    assert np.array_equal(_node_ids(outside, d), [0, len(nodes) - 1])
+
+
+def test_oracle_schedule_pins():
+    """The oracle's penalty schedule: a constant schedule equals no schedule, and the initial
+    solve under the continuation equals a plain run with penalty 1 (p(0) = 1)."""
+    from oracle import topopt as otop
+    from oracle.solvers import MultiGrid as OMG
+    from edgeto_amd.randomizer.ConfigSolve import continuation
+
+    def run(penalty, schedule, iters):
+        prob = otop.Problem((12, 6), nu=0.3, r_min=1.5)
+        prob.cons[:2 * 7] = True
+        prob.f[-1] = -1.0
+        mat = otop.SingleMaterial(penalty=penalty, volume_fraction=0.4)
+        return otop.optimize(prob, mat, OMG(prob.nel, 2, tol=1e-12, n_level=2), np.ones(prob.n_el), iters,
+                             ch_tol=0, fun_tol=0, schedule=schedule)
+
+    a = run(3.0, None, 3)[2]["comp"]
+    b = run(3.0, lambda p, i: p, 3)[2]["comp"]
+    assert a == b
+    c = run(3.0, continuation, 0)[2]["comp"]
+    d = run(1.0, None, 0)[2]["comp"]
+    assert c == d
