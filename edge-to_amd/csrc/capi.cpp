@@ -36,18 +36,23 @@ namespace {
 
 // The 3-D fine-grid kernels need an element matrix with the brick's reflection
 // structure (k0_structure); K0 arrives in device memory here, so it is read
-// back first (this synchronises the stream).  2-D grids always use the
-// general kernel.
+// back first (this synchronises the stream), and its transformed-basis table is
+// built from the same copy.  *fine = tab when the fine-grid kernels apply, else
+// null (2-D grids always use the general kernel).
 template <typename T>
-static int fine_allowed(const Grid& g, const T* K0, hipStream_t stream, bool* ok) {
-  *ok = false;
+static int fine_allowed(const Grid& g, const T* K0, hipStream_t stream, et::WhtTable* tab,
+                        const et::WhtTable** fine) {
+  *fine = nullptr;
   if (g.dim != 3) return ET_OK;
   T h[576];
   ET_CHECK_HIP(hipMemcpyAsync(h, K0, sizeof(h), hipMemcpyDeviceToHost, stream));
   ET_CHECK_HIP(hipStreamSynchronize(stream));
   double d[576];
   for (int t = 0; t < 576; ++t) d[t] = (double)h[t];
-  *ok = et::k0_fast3d(et::k0_structure(3, d, sizeof(T) == 8 ? et::kK0Rtol : et::kK0RtolF32));
+  if (et::k0_fast3d(et::k0_structure(3, d, sizeof(T) == 8 ? et::kK0Rtol : et::kK0RtolF32))) {
+    et::wht_table(d, tab);
+    *fine = tab;
+  }
   return ET_OK;
 }
 
@@ -58,8 +63,9 @@ static int residual_impl(const char* name, int dim, const int* nel, const T* K0,
   ET_TRY(et::make_grid(dim, nel, &g));
   ET_REQUIRE(K0 && E && x && b && r, name);
   ET_REQUIRE(r != x && r != b, "residual: r must not alias x or b");
-  bool fine = false;
-  ET_TRY(fine_allowed(g, K0, stream, &fine));
+  et::WhtTable tab;
+  const et::WhtTable* fine = nullptr;
+  ET_TRY(fine_allowed(g, K0, stream, &tab, &fine));
   return et::level0_op<T>(1, g, K0, E, cons, x, b, nullptr, nullptr, 1.0, nullptr, r, stream, fine);
 }
 
@@ -72,8 +78,9 @@ static int jacobi_impl(const char* name, int dim, const int* nel, const T* K0, c
   ET_REQUIRE(K0 && E && x && b && Dinv && y, name);
   ET_REQUIRE(y != x && y != b && y != Dinv, "jacobi: y must not alias x, b or Dinv");
   ET_REQUIRE(std::isfinite(omega), "jacobi: omega must be finite");
-  bool fine = false;
-  ET_TRY(fine_allowed(g, K0, stream, &fine));
+  et::WhtTable tab;
+  const et::WhtTable* fine = nullptr;
+  ET_TRY(fine_allowed(g, K0, stream, &tab, &fine));
   return et::level0_op<T>(2, g, K0, E, cons, x, b, Dinv, nullptr, omega, nullptr, y, stream, fine);
 }
 
@@ -180,8 +187,9 @@ int et_matvec_f64(int dim, const int* nel, const double* K0, const double* E, co
     ET_TRY(et::make_grid(dim, nel, &g));
     ET_REQUIRE(K0 && E && u && y, "et_matvec_f64: null pointer");
     ET_REQUIRE(u != y, "et_matvec_f64: u and y must not alias");
-    bool fine = false;
-    ET_TRY(fine_allowed(g, K0, stream, &fine));
+    et::WhtTable tab;
+    const et::WhtTable* fine = nullptr;
+    ET_TRY(fine_allowed(g, K0, stream, &tab, &fine));
     return et::matvec<double>(g, K0, E, cons, u, y, stream, fine);
   })
 }
@@ -193,8 +201,9 @@ int et_matvec_f32(int dim, const int* nel, const float* K0, const float* E, cons
     ET_TRY(et::make_grid(dim, nel, &g));
     ET_REQUIRE(K0 && E && u && y, "et_matvec_f32: null pointer");
     ET_REQUIRE(u != y, "et_matvec_f32: u and y must not alias");
-    bool fine = false;
-    ET_TRY(fine_allowed(g, K0, stream, &fine));
+    et::WhtTable tab;
+    const et::WhtTable* fine = nullptr;
+    ET_TRY(fine_allowed(g, K0, stream, &tab, &fine));
     return et::matvec<float>(g, K0, E, cons, u, y, stream, fine);
   })
 }
@@ -444,8 +453,9 @@ int et_cg_f64(int dim, const int* nel, const double* K0, const double* E, const 
     ET_REQUIRE(K0 && E && b && x && work, "et_cg_f64: null pointer");
     ET_REQUIRE(b != x, "et_cg_f64: b and x must not alias");
     ET_REQUIRE(std::isfinite(tol) && tol >= 0, "et_cg_f64: tol must be finite and >= 0");
-    bool fine = false;
-    ET_TRY(fine_allowed(g, K0, stream, &fine));
+    et::WhtTable tab;
+    const et::WhtTable* fine = nullptr;
+    ET_TRY(fine_allowed(g, K0, stream, &tab, &fine));
     return et::cg_solve(g, K0, E, cons, fine, b, x, tol, maxiter, work, work_bytes, iters, residual, stream);
   })
 }
@@ -459,8 +469,9 @@ int et_gmres_f64(int dim, const int* nel, const double* K0, const double* E, con
     ET_REQUIRE(K0 && E && b && x && work, "et_gmres_f64: null pointer");
     ET_REQUIRE(b != x, "et_gmres_f64: b and x must not alias");
     ET_REQUIRE(std::isfinite(tol) && tol >= 0, "et_gmres_f64: tol must be finite and >= 0");
-    bool fine = false;
-    ET_TRY(fine_allowed(g, K0, stream, &fine));
+    et::WhtTable tab;
+    const et::WhtTable* fine = nullptr;
+    ET_TRY(fine_allowed(g, K0, stream, &tab, &fine));
     return et::gmres_solve(g, K0, E, cons, fine, b, x, tol, restart, maxiter, work, work_bytes, iters, residual,
                            stream);
   })

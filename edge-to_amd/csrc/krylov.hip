@@ -157,19 +157,23 @@ int residual_norm(const KrylovOp& op, const double* b, const double* x, double* 
   return dot(op.n, r, r, partial, out, s);
 }
 
-// the structured operator (fine-grid kernels when `fine`) as a Krylov operator
-KrylovOp structured_op(const Grid& g, const double* K0, const double* E, const uint8_t* cons, bool fine) {
+// the structured operator (fine-grid kernels when `fine`, a copy of which the
+// operator keeps) as a Krylov operator
+KrylovOp structured_op(const Grid& g, const double* K0, const double* E, const uint8_t* cons, const WhtTable* fine) {
   KrylovOp op;
   op.n = g.n_nodes() * g.dim;
+  const bool has = fine != nullptr;
+  WhtTable tab{};
+  if (has) tab = *fine;
   op.apply = [=](int mode, const double* x, const double* b, double* y, hipStream_t s) {
-    return level0_op<double>(mode, g, K0, E, cons, x, b, nullptr, nullptr, 1.0, nullptr, y, s, fine);
+    return level0_op<double>(mode, g, K0, E, cons, x, b, nullptr, nullptr, 1.0, nullptr, y, s, has ? &tab : nullptr);
   };
   return op;
 }
 
 }  // namespace
 
-int cg_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, bool fine, const double* b,
+int cg_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, const WhtTable* fine, const double* b,
              double* x, double tol, int maxiter, void* work, int64_t work_bytes, int* iters, double* residual,
              hipStream_t s) {
   return cg_solve_op(structured_op(g, K0, E, cons, fine), b, x, tol, maxiter, work, work_bytes, iters, residual,
@@ -316,7 +320,7 @@ static void hessenberg_lstsq(int m, std::vector<double> H, double beta, double* 
   }
 }
 
-int gmres_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, bool fine,
+int gmres_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, const WhtTable* fine,
                 const double* b, double* x, double tol, int restart, int maxiter, void* work, int64_t work_bytes,
                 int* iters, double* residual, hipStream_t s) {
   return gmres_solve_op(structured_op(g, K0, E, cons, fine), b, x, tol, restart, maxiter, work, work_bytes, iters,

@@ -414,11 +414,11 @@ bool fine_supported(const Grid& g, size_t elem_bytes) {
 }
 
 template <typename T>
-int op0_fine(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E,
+int op0_fine(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const WhtTable& tab, const T* E,
              const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
              double wscale, const T* xadd, T* y, hipStream_t s) {
   if (get_wht() && wht_supported(g, sizeof(T)))
-    return op0_wht<T>(mode, g, e_sx, e_sy, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, s);
+    return op0_wht<T>(mode, g, e_sx, e_sy, tab, E, cons, x, b, Dinv, omega, wscale, xadd, y, s);
   ET_REQUIRE(mode >= 0 && mode <= 3, "fine operator: mode not available without the transformed-basis operator");
   if (get_mx(sizeof(T) == 8) && mx_supported(g, sizeof(T)))
     return op0_mx<T>(mode, g, e_sx, e_sy, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, s);
@@ -427,10 +427,10 @@ int op0_fine(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, c
 
 bool fine_jac0_supported(const Grid& g, size_t elem_bytes) { return get_wht() && wht_supported(g, elem_bytes); }
 
-template int op0_fine<double>(int, const Grid&, int64_t, int64_t, const double*, const double*,
+template int op0_fine<double>(int, const Grid&, int64_t, int64_t, const double*, const WhtTable&, const double*,
                               const uint8_t*, const double*, const double*, const double*,
                               const double*, double, const double*, double*, hipStream_t);
-template int op0_fine<float>(int, const Grid&, int64_t, int64_t, const float*, const float*,
+template int op0_fine<float>(int, const Grid&, int64_t, int64_t, const float*, const WhtTable&, const float*,
                              const uint8_t*, const float*, const float*, const float*,
                              const double*, double, const float*, float*, hipStream_t);
 

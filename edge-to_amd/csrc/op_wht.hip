@@ -30,6 +30,12 @@
 // LDS and the node phase adds the contributions of element columns i and
 // i-1 (and the seam slot at k = 64, 128).
 //
+// Coefficients.  G's 45 non-zeros depend on K0 only: the host builds them once
+// (wht_table) and every launch takes them as a by-value kernel argument, which
+// the contraction reads with scalar loads per output component (held in scalar
+// registers for the whole kernel they do not fit next to the other scalars and
+// spill to vector lanes).
+//
 // Tiling.  A workgroup (8 wavefronts) owns TC element columns in x (16 for
 // nelz <= 64, 8 for nelz <= 128; all z), completing TC-1 node columns, and
 // marches in y; the (x tile, node plane) units are split evenly over a
@@ -79,9 +85,44 @@ __host__ inline size_t wht_lds_values(const Grid& g, int ns) {
   return 6 * span                 // two staged node planes
          + 6 * (size_t)tc * nz    // node contributions [3][2][tc][nz]
          + 6 * (size_t)tc * 2     // seam slots [3][2][tc][2]
-         + (size_t)tc * g.ez      // E of one layer
-         + 64                     // G
-         + 576;                   // K0, staged once to form G
+         + (size_t)tc * g.ez;     // E of one layer
+}
+
+// G's non-zeros in contraction order, 16 per output component (the last one 0);
+// the kernel's first argument, so that it sits at offset 0 of the kernel argument
+// segment
+template <typename T>
+struct WhtK {
+  T g[48];
+};
+static_assert(sizeof(WhtK<double>) == 384 && sizeof(WhtK<float>) == 192, "coefficient table layout");
+
+void wht_table(const double* K0, WhtTable* t) {
+  double G[kWG];
+  for (int q = 0; q < kWG; ++q) {
+    const int M = 1 + q / 9, c = (q / 3) % 3, d = q % 3;
+    const int Mp = M ^ (1 << c) ^ (1 << d);
+    double s = 0.0;
+    for (int bb = 0; bb < 8; ++bb) {
+      const int a = h8_corner(bb & 1, (bb >> 1) & 1, bb >> 2);
+      for (int bp = 0; bp < 8; ++bp) {
+        const int ap = h8_corner(bp & 1, (bp >> 1) & 1, bp >> 2);
+        const int sg1 = __builtin_popcount(M & bb) & 1, sg2 = __builtin_popcount(Mp & bp) & 1;
+        const double k = K0[(3 * a + c) * 24 + 3 * ap + d];
+        s += (sg1 ^ sg2) ? -k : k;
+      }
+    }
+    G[q] = s / 64.0;
+  }
+  // G is symmetric, G[M][c][d] = G[M ^ e_c ^ e_d][d][c]: the entries with c <= d are used
+  for (int c = 0; c < 3; ++c) {
+    int i = 0;
+    for (int M = 1; M < 8; ++M)
+      for (int d = 0; d < 3; ++d)
+        if (wht_nz(M, c, d))
+          t->g[16 * c + i++] = c <= d ? G[(M - 1) * 9 + c * 3 + d] : G[((M ^ (1 << c) ^ (1 << d)) - 1) * 9 + d * 3 + c];
+    for (; i < 16; ++i) t->g[16 * c + i] = 0.0;
+  }
 }
 
 // lane j takes lane j-1 of the wavefront, lane 0 takes 0
@@ -124,7 +165,7 @@ constexpr int wht_min_waves() {
 
 template <typename T, int MODE, int NT, int NS>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(wht_min_waves<T, NT, NS>())))
-void op0_wht_kernel(const MxArgs A, const T* __restrict__ K0,
+void op0_wht_kernel(const WhtK<T> Gk, const MxArgs A,
                                                             const T* __restrict__ E,
                                                             const uint8_t* __restrict__ cons,
                                                             const T* __restrict__ x,
@@ -158,7 +199,6 @@ void op0_wht_kernel(const MxArgs A, const T* __restrict__ K0,
   T* sw = su + 6 * span;                  // [3 comps][2 hx][tc][nz]
   T* ss = sw + 6 * tc * nz;               // seams [3 comps][2 hx][tc][2]
   T* se = ss + 12 * tc;                   // [tc][nelz]
-  T* sg = se + tc * nelz;                 // G, 63 entries
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wave_u = __builtin_amdgcn_readfirstlane(wave);
   // 32-bit DOF indices and byte offsets: the host checks 3 * n_nodes < 2^31 and 3 * n_nodes * sizeof(T) < 2^32
@@ -177,41 +217,6 @@ void op0_wht_kernel(const MxArgs A, const T* __restrict__ K0,
   constexpr int kWEStage = Cfg::estage;
   constexpr int kWNodes = Cfg::nodes;
 
-  // G[M][c][d] from K0 (corner order of K0 is the H8 local order).  K0 is staged
-  // in LDS by the whole workgroup first (one coalesced load instead of a chain of
-  // dependent loads per thread).
-  T* sk0 = sg + 64;
-  for (int t = tid; t < 576; t += NT) sk0[t] = K0[t];
-  __syncthreads();
-  if (tid < kWG) {
-    const int M = 1 + tid / 9, c = (tid / 3) % 3, d = tid % 3;
-    const int Mp = M ^ (1 << c) ^ (1 << d);
-    double s = 0.0;
-    for (int bb = 0; bb < 8; ++bb) {
-      const int a = h8_corner(bb & 1, (bb >> 1) & 1, bb >> 2);
-      for (int bp = 0; bp < 8; ++bp) {
-        const int ap = h8_corner(bp & 1, (bp >> 1) & 1, bp >> 2);
-        const int sg1 = __builtin_popcount(M & bb) & 1, sg2 = __builtin_popcount(Mp & bp) & 1;
-        const double k = (double)sk0[(3 * a + c) * 24 + 3 * ap + d];
-        s += (sg1 ^ sg2) ? -k : k;
-      }
-    }
-    sg[tid] = (T)(s / 64.0);
-  }
-  __syncthreads();
-  // The 45 non-zero coefficients are wave-uniform: scalar registers.  G is
-  // symmetric, G[M][c][d] = G[M ^ e_c ^ e_d][d][c], so only the 33 entries with
-  // c <= d are kept (fewer scalar registers to spill to vector lanes).
-  T gk[kWG];
-#pragma unroll
-  for (int t = 0; t < kWG; ++t) {
-    const int M = 1 + t / 9, c = (t / 3) % 3, d = t % 3;
-    gk[t] = T(0);
-    if (wht_nz(M, c, d) && c <= d) gk[t] = wht_uniform(sg[t]);
-  }
-  auto gval = [&](int M, int c, int d) -> T {
-    return c <= d ? gk[(M - 1) * 9 + c * 3 + d] : gk[((M ^ (1 << c) ^ (1 << d)) - 1) * 9 + d * 3 + c];
-  };
   const T w = kJac ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
   double dot = 0.0;  // this thread's share of x.y (TM_YDOT) or dotv.y (TM_JAC_ADD_DOT)
 
@@ -411,14 +416,22 @@ void op0_wht_kernel(const MxArgs A, const T* __restrict__ K0,
         const int k = ez;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
+          // this component's 15 coefficients: scalar loads from the kernel argument
+          // segment (Gk is the first argument, at offset 0; taking Gk's address
+          // instead would make a private copy), through a pointer the compiler may
+          // not hoist out of the layer loop
+          const __attribute__((address_space(4))) T* gq =
+              (const __attribute__((address_space(4))) T*)__builtin_amdgcn_kernarg_segment_ptr() + 16 * c;
+          asm volatile("" : "+s"(gq));
           T yc[8];
           yc[0] = T(0);
+          int gi = 0;
 #pragma unroll
           for (int M = 1; M < 8; ++M) {
             T s = T(0);
 #pragma unroll
             for (int d = 0; d < 3; ++d)
-              if (wht_nz(M, c, d)) s = fma(gval(M, c, d), uh[d][M ^ (1 << c) ^ (1 << d)], s);
+              if (wht_nz(M, c, d)) s = fma(gq[gi++], uh[d][M ^ (1 << c) ^ (1 << d)], s);
             yc[M] = s;
           }
           // y stage of the inverse; this layer's bottom face completes node plane ey
@@ -602,7 +615,7 @@ static int wht_device_cus() {
 }
 
 template <typename T, int MODE, int NT, int NS>
-static void launch_wht_cfg(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const T* K0, const T* E,
+static void launch_wht_cfg(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const WhtK<T>& Gk, const T* E,
                            const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
                            double wscale, const T* xadd, T* y, double* partial, const T* dotv) {
   static bool attr_set = false;
@@ -611,28 +624,36 @@ static void launch_wht_cfg(int n_wg, size_t lds, hipStream_t s, const MxArgs& A,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWMaxLds);
     attr_set = true;
   }
-  op0_wht_kernel<T, MODE, NT, NS><<<n_wg, NT, lds, s>>>(A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y,
+  op0_wht_kernel<T, MODE, NT, NS><<<n_wg, NT, lds, s>>>(Gk, A, E, cons, x, b, Dinv, omega, wscale, xadd, y,
                                                         partial, dotv);
 }
 
 // 16 slices: 1024 or 512 threads; 8 slices: 512 or 256 threads
 template <typename T, int MODE>
-static void launch_wht(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const T* K0, const T* E,
+static void launch_wht(int n_wg, size_t lds, hipStream_t s, const MxArgs& A, const WhtK<T>& Gk, const T* E,
                        const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
                        double wscale, const T* xadd, T* y, double* partial = nullptr,
                        const T* dotv = nullptr) {
   const int p = sizeof(T) == 8 ? 1 : 0;
   if (wht_ns(sizeof(T), MODE) == 16) {
     if (wht_nt_eff(p) == 1024)
-      launch_wht_cfg<T, MODE, 1024, 16>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
+      launch_wht_cfg<T, MODE, 1024, 16>(n_wg, lds, s, A, Gk, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
     else
-      launch_wht_cfg<T, MODE, 512, 16>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
+      launch_wht_cfg<T, MODE, 512, 16>(n_wg, lds, s, A, Gk, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
   } else {
     if (wht_nt_eff(p) == 256)
-      launch_wht_cfg<T, MODE, 256, 8>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
+      launch_wht_cfg<T, MODE, 256, 8>(n_wg, lds, s, A, Gk, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
     else
-      launch_wht_cfg<T, MODE, 512, 8>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
+      launch_wht_cfg<T, MODE, 512, 8>(n_wg, lds, s, A, Gk, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
   }
+}
+
+// the table in the kernel's precision
+template <typename T>
+static WhtK<T> wht_k(const WhtTable& t) {
+  WhtK<T> k;
+  for (int i = 0; i < 48; ++i) k.g[i] = (T)t.g[i];
+  return k;
 }
 
 // x tiles, persistent grid (ET_TUNE_WHT_WGS workgroups per compute unit for
@@ -662,66 +683,69 @@ int wht_partials(const Grid& g, size_t elem_bytes, int mode) {
 }
 
 template <typename T>
-int op0_wht_ydot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E, const uint8_t* cons,
+int op0_wht_ydot(const Grid& g, int64_t e_sx, int64_t e_sy, const WhtTable& tab, const T* E, const uint8_t* cons,
                  const T* x, T* y, double* partial, int max_partial, hipStream_t s) {
   ET_REQUIRE(wht_supported(g, sizeof(T)), "transformed-basis operator: 3-D grids with nelz <= 128 only");
   int n_wg = 0;
   size_t lds = 0;
   const MxArgs A = wht_args(g, e_sx, e_sy, sizeof(T), TM_YDOT, &n_wg, &lds);
+  const WhtK<T> Gk = wht_k<T>(tab);
   ET_REQUIRE(partial != nullptr && n_wg <= max_partial, "transformed-basis operator: partial buffer too small");
-  launch_wht<T, TM_YDOT>(n_wg, lds, s, A, K0, E, cons, x, nullptr, nullptr, nullptr, 1.0, nullptr, y, partial);
+  launch_wht<T, TM_YDOT>(n_wg, lds, s, A, Gk, E, cons, x, nullptr, nullptr, nullptr, 1.0, nullptr, y, partial);
   ET_CHECK_LAUNCH();
   return ET_OK;
 }
-template int op0_wht_ydot<double>(const Grid&, int64_t, int64_t, const double*, const double*, const uint8_t*,
+template int op0_wht_ydot<double>(const Grid&, int64_t, int64_t, const WhtTable&, const double*, const uint8_t*,
                                   const double*, double*, double*, int, hipStream_t);
 
 template <typename T>
-int op0_wht_jac_add_dot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E, const uint8_t* cons,
+int op0_wht_jac_add_dot(const Grid& g, int64_t e_sx, int64_t e_sy, const WhtTable& tab, const T* E, const uint8_t* cons,
                         const T* x, const T* b, const T* Dinv, const double* omega, double wscale, const T* xadd,
                         T* y, const T* dotv, double* partial, int max_partial, hipStream_t s) {
   ET_REQUIRE(wht_supported(g, sizeof(T)), "transformed-basis operator: 3-D grids with nelz <= 128 only");
   int n_wg = 0;
   size_t lds = 0;
   const MxArgs A = wht_args(g, e_sx, e_sy, sizeof(T), TM_JAC_ADD_DOT, &n_wg, &lds);
+  const WhtK<T> Gk = wht_k<T>(tab);
   ET_REQUIRE(partial != nullptr && n_wg <= max_partial, "transformed-basis operator: partial buffer too small");
   ET_REQUIRE(dotv != nullptr && xadd != nullptr, "transformed-basis operator: null operand");
-  launch_wht<T, TM_JAC_ADD_DOT>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
+  launch_wht<T, TM_JAC_ADD_DOT>(n_wg, lds, s, A, Gk, E, cons, x, b, Dinv, omega, wscale, xadd, y, partial, dotv);
   ET_CHECK_LAUNCH();
   return ET_OK;
 }
-template int op0_wht_jac_add_dot<float>(const Grid&, int64_t, int64_t, const float*, const float*, const uint8_t*,
+template int op0_wht_jac_add_dot<float>(const Grid&, int64_t, int64_t, const WhtTable&, const float*, const uint8_t*,
                                         const float*, const float*, const float*, const double*, double,
                                         const float*, float*, const float*, double*, int, hipStream_t);
-template int op0_wht_jac_add_dot<double>(const Grid&, int64_t, int64_t, const double*, const double*,
+template int op0_wht_jac_add_dot<double>(const Grid&, int64_t, int64_t, const WhtTable&, const double*,
                                          const uint8_t*, const double*, const double*, const double*,
                                          const double*, double, const double*, double*, const double*, double*,
                                          int, hipStream_t);
 
 template <typename T>
-int op0_wht(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E,
+int op0_wht(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const WhtTable& tab, const T* E,
             const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
             double wscale, const T* xadd, T* y, hipStream_t s) {
   ET_REQUIRE(wht_supported(g, sizeof(T)), "transformed-basis operator: 3-D grids with nelz <= 128 only");
   int n_wg = 0;
   size_t lds = 0;
   const MxArgs A = wht_args(g, e_sx, e_sy, sizeof(T), mode, &n_wg, &lds);
+  const WhtK<T> Gk = wht_k<T>(tab);
   switch (mode) {
-    case TM_Y: launch_wht<T, TM_Y>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
-    case TM_RES: launch_wht<T, TM_RES>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
-    case TM_JAC: launch_wht<T, TM_JAC>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
-    case TM_JAC_ADD: launch_wht<T, TM_JAC_ADD>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
-    case TM_JAC0: launch_wht<T, TM_JAC0>(n_wg, lds, s, A, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
+    case TM_Y: launch_wht<T, TM_Y>(n_wg, lds, s, A, Gk, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
+    case TM_RES: launch_wht<T, TM_RES>(n_wg, lds, s, A, Gk, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
+    case TM_JAC: launch_wht<T, TM_JAC>(n_wg, lds, s, A, Gk, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
+    case TM_JAC_ADD: launch_wht<T, TM_JAC_ADD>(n_wg, lds, s, A, Gk, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
+    case TM_JAC0: launch_wht<T, TM_JAC0>(n_wg, lds, s, A, Gk, E, cons, x, b, Dinv, omega, wscale, xadd, y); break;
     default: ET_REQUIRE(false, "transformed-basis operator: unknown mode");
   }
   ET_CHECK_LAUNCH();
   return ET_OK;
 }
 
-template int op0_wht<double>(int, const Grid&, int64_t, int64_t, const double*, const double*,
+template int op0_wht<double>(int, const Grid&, int64_t, int64_t, const WhtTable&, const double*,
                              const uint8_t*, const double*, const double*, const double*,
                              const double*, double, const double*, double*, hipStream_t);
-template int op0_wht<float>(int, const Grid&, int64_t, int64_t, const float*, const float*,
+template int op0_wht<float>(int, const Grid&, int64_t, int64_t, const WhtTable&, const float*,
                             const uint8_t*, const float*, const float*, const float*,
                             const double*, double, const float*, float*, hipStream_t);
 

@@ -449,7 +449,7 @@ static int op_launch(const MGLevel<T>& L, const MGShared& sh, int mode, const T*
     return st;
   }
   if (L.level == 0 && sh.tiled) {
-    const int st = op0_fine<T>(mode, g, (int64_t)g.ey * g.ez, g.ez, sh.K0<T>(), sh.E<T>(), sh.cons,
+    const int st = op0_fine<T>(mode, g, (int64_t)g.ey * g.ez, g.ez, sh.K0<T>(), sh.wht, sh.E<T>(), sh.cons,
                                 x, b, L.Dinv, omega, wscale, xadd, y, s);
     sh.prof->end(s);
     return st;
@@ -552,7 +552,7 @@ int MGHierarchy<T>::vcycle(int l, MGShared& sh, hipStream_t s, bool x0_ready) {
       } else if (l == 0 && pass + 1 == passes && sh.want_dot) {
         // the preconditioned residual z is final here: r.z partials in the same pass
         sh.prof->begin(ET_PROF_L0_JACADD, s);
-        const int st = op0_wht_jac_add_dot<T>(L.g, (int64_t)L.g.ey * L.g.ez, L.g.ez, sh.K0<T>(), sh.E<T>(),
+        const int st = op0_wht_jac_add_dot<T>(L.g, (int64_t)L.g.ey * L.g.ez, L.g.ez, sh.wht, sh.E<T>(),
                                               sh.cons, L.e, L.r, L.Dinv, om, ws, L.x, L.t, L.b,
                                               sh.dot_part, kRedBlocks, s);
         sh.prof->end(s);
@@ -607,9 +607,10 @@ template struct MGHierarchy<float>;
 template <typename T>
 int level0_op(int mode, const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T* x,
               const T* b, const T* Dinv, const double* omega, double wscale, const T* xadd, T* y,
-              hipStream_t s, bool allow_fine) {
-  if (g.dim == 3 && allow_fine && fine_supported(g, sizeof(T)))
-    return op0_fine<T>(mode, g, (int64_t)g.ey * g.ez, g.ez, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y, s);
+              hipStream_t s, const WhtTable* fine) {
+  if (g.dim == 3 && fine != nullptr && fine_supported(g, sizeof(T)))
+    return op0_fine<T>(mode, g, (int64_t)g.ey * g.ez, g.ez, K0, *fine, E, cons, x, b, Dinv, omega, wscale, xadd, y,
+                       s);
   const unsigned grid = grid_for(g.n_nodes());
 #define ET_L0(DIM, MODE) \
   op0_kernel<T, DIM, MODE><<<grid, kBlock, 0, s>>>(g, K0, E, cons, x, b, Dinv, omega, wscale, xadd, y)
@@ -634,10 +635,10 @@ int level0_op(int mode, const Grid& g, const T* K0, const T* E, const uint8_t* c
 
 template int level0_op<double>(int, const Grid&, const double*, const double*, const uint8_t*,
                                const double*, const double*, const double*, const double*, double,
-                               const double*, double*, hipStream_t, bool);
+                               const double*, double*, hipStream_t, const WhtTable*);
 template int level0_op<float>(int, const Grid&, const float*, const float*, const uint8_t*,
                               const float*, const float*, const float*, const double*, double,
-                              const float*, float*, hipStream_t, bool);
+                              const float*, float*, hipStream_t, const WhtTable*);
 
 // ------------------------------------------------------------ et_mg ----
 
@@ -706,6 +707,7 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
   sh.tiled = g0.dim == 3 && k0_fast3d(k0_flags) && fine_supported(g0, sizeof(double)) &&
              fine_supported(g0, sizeof(float));
   sh.mfma = p.precision == 1 && sh.tiled && get_mixed_mfma() && mfma_supported(g0);
+  if (sh.tiled) wht_table(K0_host, &sh.wht);
   sh.kscale = k0_scale(K0_host, (p.dim == 3 ? 24 * 24 : 8 * 8));
   const int nd = (p.dim == 3 ? 8 : 4) * dof;
   ET_CHECK_HIP(hipMalloc(&sh.K0d, nd * nd * sizeof(double)));
@@ -854,7 +856,7 @@ int MGSolver::apply_A64(const double* x, double* y, int mode, const double* b) {
   const unsigned grid = grid_for(g0.n_nodes());
   prof.begin(ET_PROF_MATVEC64, s);
   if (sh.tiled) {
-    const int st = op0_fine<double>(mode, g0, (int64_t)g0.ey * g0.ez, g0.ez, sh.K0d, sh.Ed, sh.cons,
+    const int st = op0_fine<double>(mode, g0, (int64_t)g0.ey * g0.ez, g0.ez, sh.K0d, sh.wht, sh.Ed, sh.cons,
                                      x, b, nullptr, sh.scal, 1.0, nullptr, y, s);
     prof.end(s);
     return st;
@@ -963,7 +965,7 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
     if (fused_pq) {
       // q = A p with p.q summed per workgroup inside the operator
       prof.begin(ET_PROF_MATVEC64, s);
-      ET_TRY(op0_wht_ydot<double>(g0, (int64_t)g0.ey * g0.ez, g0.ez, sh.K0d, sh.Ed, sh.cons, p_, q, partial,
+      ET_TRY(op0_wht_ydot<double>(g0, (int64_t)g0.ey * g0.ez, g0.ez, sh.wht, sh.Ed, sh.cons, p_, q, partial,
                                   kRedBlocks, s));
       prof.end(s);
       n_pq = n_pq_fused;

@@ -5,6 +5,7 @@
 #include "coarse.hpp"
 #include "common.hpp"
 #include "level1.hpp"
+#include "stiffness.hpp"
 
 namespace et {
 
@@ -63,6 +64,7 @@ struct MGShared {
   bool tiled = false;  // fine grid uses the LDS-tiled H8 operator
   bool mfma = false;   // FP32 fine-grid sweeps use the matrix-core operator
   float kscale = 1.f;  // power-of-two scale of K0 for the FP16 split
+  WhtTable wht{};      // transformed-basis coefficients of K0 (when tiled)
   double* K0d = nullptr;
   float* K0f = nullptr;
   const double* Ed = nullptr;  // caller's E (FP64)

@@ -116,10 +116,10 @@ __global__ __launch_bounds__(256) void matvec_kernel(const Grid g, const T* __re
 
 template <typename T>
 int matvec(const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T* u, T* y,
-           hipStream_t s, bool allow_fine) {
+           hipStream_t s, const WhtTable* fine) {
   const unsigned grid = grid_for(g.n_nodes());
-  if (g.dim == 3 && allow_fine && fine_supported(g, sizeof(T)))
-    return op0_fine<T>(0, g, (int64_t)g.ey * g.ez, g.ez, K0, E, cons, u, nullptr, nullptr, nullptr,
+  if (g.dim == 3 && fine != nullptr && fine_supported(g, sizeof(T)))
+    return op0_fine<T>(0, g, (int64_t)g.ey * g.ez, g.ez, K0, *fine, E, cons, u, nullptr, nullptr, nullptr,
                         1.0, nullptr, y, s);
   if (g.dim == 3)
     matvec_kernel<T, 3><<<grid, kBlock, 0, s>>>(g, K0, E, cons, u, y);
@@ -129,9 +129,9 @@ int matvec(const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T*
   return ET_OK;
 }
 template int matvec<double>(const Grid&, const double*, const double*, const uint8_t*,
-                            const double*, double*, hipStream_t, bool);
+                            const double*, double*, hipStream_t, const WhtTable*);
 template int matvec<float>(const Grid&, const float*, const float*, const uint8_t*, const float*,
-                           float*, hipStream_t, bool);
+                           float*, hipStream_t, const WhtTable*);
 
 // -------------------------------------------------------------- diagonal ---
 

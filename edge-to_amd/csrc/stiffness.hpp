@@ -23,11 +23,20 @@ inline bool k0_fast3d(int flags) {
   return (flags & (K0S_SYM | K0S_REFLECT | K0S_WHT)) == (K0S_SYM | K0S_REFLECT | K0S_WHT);
 }
 
-// allow_fine = false keeps 3-D grids on the general node-gather kernel (element
-// matrices without the reflection structure, k0_fast3d)
+// Coefficients of the transformed-basis fine-grid operator for one element matrix
+// (op_wht.hip): the 45 non-zeros of G = H K0 H / 64, 15 per output component in
+// the order its contraction uses them, each component padded to 16.  Built on the
+// host (wht_table) by whoever holds K0 there; a null table stands for an element
+// matrix without the brick's reflection structure (k0_fast3d false), which keeps 3-D
+// grids on the general node-gather kernels.
+struct WhtTable {
+  double g[48];
+};
+void wht_table(const double* K0_host, WhtTable* t);
+
 template <typename T>
 int matvec(const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T* u, T* y,
-           hipStream_t s, bool allow_fine = true);
+           hipStream_t s, const WhtTable* fine = nullptr);
 template <typename T>
 int diagonal(const Grid& g, const T* K0, const T* E, const uint8_t* cons, T* diag, hipStream_t s);
 
@@ -51,21 +60,21 @@ void set_mx_wgs(int is_double, int n);
 int get_mx_wgs(int is_double);
 // Transformed-basis (corner Walsh-Hadamard) element operator, FP32 or FP64, same modes.
 template <typename T>
-int op0_wht(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E,
+int op0_wht(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const WhtTable& tab, const T* E,
             const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
             double wscale, const T* xadd, T* y, hipStream_t s);
 bool wht_supported(const Grid& g, size_t elem_bytes);
 // y = A x (FP64) fused with the x.y partial sum of every workgroup (fixed order);
 // writes wht_partials(g, sizeof(T), TM_YDOT) <= max_partial values to partial.
 template <typename T>
-int op0_wht_ydot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E, const uint8_t* cons,
+int op0_wht_ydot(const Grid& g, int64_t e_sx, int64_t e_sy, const WhtTable& tab, const T* E, const uint8_t* cons,
                  const T* x, T* y, double* partial, int max_partial, hipStream_t s);
 // number of per-workgroup partial sums written by the fused-dot variants
 int wht_partials(const Grid& g, size_t elem_bytes, int mode);
 // Final post-smoothing sweep y = xadd + Jacobi(x; b) with the per-workgroup
 // partial sums of dotv.y (wht_partials(g, sizeof(T), TM_JAC_ADD_DOT) values).
 template <typename T>
-int op0_wht_jac_add_dot(const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E, const uint8_t* cons,
+int op0_wht_jac_add_dot(const Grid& g, int64_t e_sx, int64_t e_sy, const WhtTable& tab, const T* E, const uint8_t* cons,
                         const T* x, const T* b, const T* Dinv, const double* omega, double wscale, const T* xadd,
                         T* y, const T* dotv, double* partial, int max_partial, hipStream_t s);
 void set_wht(int on);
@@ -80,8 +89,9 @@ void set_thread_wht_tiles(const int* nt, const int* ns);
 int get_wht_slices(int is_double);
 // Fine-grid dispatch: the transformed-basis operator where enabled and
 // supported, else the matrix-core one where enabled, else the LDS-tiled one.
+// tab: the element matrix's table (it has the reflection structure).
 template <typename T>
-int op0_fine(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const T* E,
+int op0_fine(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, const T* K0, const WhtTable& tab, const T* E,
              const uint8_t* cons, const T* x, const T* b, const T* Dinv, const double* omega,
              double wscale, const T* xadd, T* y, hipStream_t s);
 // whether op0_fine supports the fused zero-guess Jacobi mode (5) on this grid
@@ -92,7 +102,7 @@ bool fine_supported(const Grid& g, size_t elem_bytes);
 template <typename T>
 int level0_op(int mode, const Grid& g, const T* K0, const T* E, const uint8_t* cons, const T* x,
               const T* b, const T* Dinv, const double* omega, double wscale, const T* xadd, T* y,
-              hipStream_t s, bool allow_fine = true);
+              hipStream_t s, const WhtTable* fine = nullptr);
 // Matrix-core FP32 variant (FP16 hi/lo MFMA); kscale: power of two >= max|K0|.
 int op0_mfma(int mode, const Grid& g, int64_t e_sx, int64_t e_sy, float kscale, const float* K0,
              const float* E, const uint8_t* cons, const float* x, const float* b, const float* Dinv,
@@ -187,10 +197,10 @@ int cg_solve_op(const KrylovOp& op, const double* b, double* x, double tol, int 
                 int64_t work_bytes, int* iters, double* residual, hipStream_t s);
 int gmres_solve_op(const KrylovOp& op, const double* b, double* x, double tol, int restart, int maxiter,
                    void* work, int64_t work_bytes, int* iters, double* residual, hipStream_t s);
-int cg_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, bool fine, const double* b,
+int cg_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, const WhtTable* fine, const double* b,
              double* x, double tol, int maxiter, void* work, int64_t work_bytes, int* iters, double* residual,
              hipStream_t s);
-int gmres_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, bool fine,
+int gmres_solve(const Grid& g, const double* K0, const double* E, const uint8_t* cons, const WhtTable* fine,
                 const double* b, double* x, double tol, int restart, int maxiter, void* work, int64_t work_bytes,
                 int* iters, double* residual, hipStream_t s);
 
