@@ -19,14 +19,13 @@
 // Passes per application, no atomics, fixed summation order:
 //   1. one thread per coarse element computes its 8 corner results; the top
 //      (bz = 1) ones move one lane up to the element above, so each element
-//      stores the 4 node sums of its bottom face (12 values, plane by plane).
-//      Extra workgroups of the same launch compute, one thread per (corrected
-//      node, component), the 27 correction blocks' product dA x (it depends
-//      on x only);
+//      stores the 4 node sums of its bottom face (12 values, plane by plane);
 //   2. one thread per coarse node sums the bottom-face sums of its (up to 4)
-//      element columns, scales by 1/tw, adds the correction product where the
-//      node has one and applies the mode epilogue (y = Ax, b - Ax, Jacobi,
-//      Jacobi + add).
+//      element columns, scales by 1/tw and applies the mode epilogue (y = Ax,
+//      b - Ax, Jacobi, Jacobi + add); nodes with a correction row store the
+//      raw sum instead;
+//   3. one thread per (corrected node, component) adds its 27 correction
+//      blocks and applies the epilogue.
 #include <cmath>
 #include <vector>
 
@@ -198,48 +197,12 @@ __device__ __forceinline__ double l1_shr(double v) {
 // element the 4 node sums (bx, by) x 3 of its bottom face; results that cannot
 // move (lane 63, the top element of a column) are stored in top instead.
 // Plane (bx + 2 by) * 3 + c of lo / top, element index C.
-// Correction product of (corrected node alist[a], component r), thread t = r * n_aff + a:
-// corr[t] = sum over the 27 neighbours m and components c of dA[o][r][c][a] x[m][c].
-template <typename T>
-__device__ __forceinline__ void l1_corr(const Grid& g1, int t, const int* __restrict__ alist, int n_aff,
-                                        const T* __restrict__ dA, const T* __restrict__ x, T* __restrict__ corr) {
-  if (t >= n_aff * 3) return;
-  const int r = t / n_aff, a = t - r * n_aff;
-  const int n = alist[a];
-  const int nx = g1.nx(), ny = g1.ny(), nz = g1.nz();
-  const int k = n % nz, ij = n / nz, i = ij % nx, j = ij / nx;
-  T acc = T(0);
-  // all 81 products are independent: out-of-grid neighbours read the node
-  // itself and are weighted by zero, so every load can be in flight at once
-#pragma unroll
-  for (int o = 0; o < 27; ++o) {
-    const int mi = i + o / 9 - 1, mj = j + (o / 3) % 3 - 1, mk = k + o % 3 - 1;
-    const bool in = mi >= 0 && mi < nx && mj >= 0 && mj < ny && mk >= 0 && mk < nz;
-    const int m = in ? (mj * nx + mi) * nz + mk : n;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const T coef = dA[(int64_t)((o * 3 + r) * 3 + c) * n_aff + a];
-      acc += (in ? coef : T(0)) * x[(int64_t)m * 3 + c];
-    }
-  }
-  corr[t] = acc;
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void l1_elem_kernel(const Grid g1, const T* __restrict__ cf,
                                                       const T* __restrict__ Eh, const T* __restrict__ x,
-                                                      T* __restrict__ lo, T* __restrict__ top, int corr_blocks,
-                                                      const int* __restrict__ alist, int n_aff,
-                                                      const T* __restrict__ dA, T* __restrict__ corr) {
-  if ((int)blockIdx.x < corr_blocks) {
-    // the first workgroups compute the correction products (whole workgroups, so
-    // no element wavefront loses lanes for the lane shift below); dispatched
-    // first, their dependent loads overlap the element workgroups
-    l1_corr<T>(g1, (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x, alist, n_aff, dA, x, corr);
-    return;
-  }
+                                                      T* __restrict__ lo, T* __restrict__ top) {
   const int ne = (int)g1.n_elem();
-  const int C0 = ((int)blockIdx.x - corr_blocks) * blockDim.x + threadIdx.x;
+  const int C0 = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = C0 < ne;
   const int C = valid ? C0 : ne - 1;  // every lane takes part in the lane shift
   const int lane = threadIdx.x & 63;
@@ -284,12 +247,13 @@ __global__ __launch_bounds__(256) void l1_elem_kernel(const Grid g1, const T* __
 
 // Node pass: sum of the (up to 4) element columns' bottom-face sums, plus the
 // unmoved top results of the element below where there are any; scale by
-// 1/tw; plus the correction product on corrected nodes; epilogue.
+// 1/tw; epilogue.  Nodes with a correction row store the raw sum (finished by
+// l1_fix_kernel).
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void l1_node_kernel(const Grid g1, const T* __restrict__ lo,
                                                       const T* __restrict__ top,
                                                       const int* __restrict__ aidx, int n_aff,
-                                                      const T* __restrict__ corr, const T* __restrict__ x,
+                                                      const T* __restrict__ dA, const T* __restrict__ x,
                                                       const T* __restrict__ b, const T* __restrict__ Dinv,
                                                       const double* __restrict__ omega, double wscale,
                                                       const T* __restrict__ xadd, T* __restrict__ y) {
@@ -322,10 +286,11 @@ __global__ __launch_bounds__(256) void l1_node_kernel(const Grid g1, const T* __
   const T s = l1_inv_tw_t<T>(g1, i, j, k);
 #pragma unroll
   for (int r = 0; r < 3; ++r) acc[r] *= s;
-  const int a = aidx[n];
-  if (a >= 0) {
+  if (aidx[n] >= 0) {
+    // corrected node: the raw product goes to y, l1_fix_kernel finishes it
 #pragma unroll
-    for (int r = 0; r < 3; ++r) acc[r] += corr[r * n_aff + a];
+    for (int r = 0; r < 3; ++r) y[(int64_t)n * 3 + r] = acc[r];
+    return;
   }
   const T w = MODE >= 2 ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
 #pragma unroll
@@ -341,6 +306,46 @@ __global__ __launch_bounds__(256) void l1_node_kernel(const Grid g1, const T* __
     if (MODE == 3) v += xadd[d];
     y[d] = v;
   }
+}
+
+// Corrected nodes: y = epilogue(raw + dA x), one thread per (listed node, component).
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void l1_fix_kernel(const Grid g1, const int* __restrict__ alist, int n_aff,
+                                                     const T* __restrict__ dA, const T* __restrict__ x,
+                                                     const T* __restrict__ b, const T* __restrict__ Dinv,
+                                                     const double* __restrict__ omega, double wscale,
+                                                     const T* __restrict__ xadd, T* __restrict__ y) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_aff * 3) return;
+  const int r = t / n_aff, a = t - r * n_aff;
+  const int n = alist[a];
+  const int nx = g1.nx(), ny = g1.ny(), nz = g1.nz();
+  const int k = n % nz, ij = n / nz, i = ij % nx, j = ij / nx;
+  const int64_t d = (int64_t)n * 3 + r;
+  T acc = y[d];
+  // all 81 products are independent: out-of-grid neighbours read the node
+  // itself and are weighted by zero, so every load can be in flight at once
+#pragma unroll
+  for (int o = 0; o < 27; ++o) {
+    const int mi = i + o / 9 - 1, mj = j + (o / 3) % 3 - 1, mk = k + o % 3 - 1;
+    const bool in = mi >= 0 && mi < nx && mj >= 0 && mj < ny && mk >= 0 && mk < nz;
+    const int m = in ? (mj * nx + mi) * nz + mk : n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T coef = dA[(int64_t)((o * 3 + r) * 3 + c) * n_aff + a];
+      acc += (in ? coef : T(0)) * x[(int64_t)m * 3 + c];
+    }
+  }
+  const T w = MODE >= 2 ? (T)((omega != nullptr ? *omega : 1.0) * wscale) : T(0);
+  T v;
+  if (MODE == 0)
+    v = acc;
+  else if (MODE == 1)
+    v = b[d] - acc;
+  else
+    v = x[d] + w * Dinv[d] * (b[d] - acc);
+  if (MODE == 3) v += xadd[d];
+  y[d] = v;
 }
 
 // ------------------------------------------------------------ host ----
@@ -445,14 +450,11 @@ int Level1MF::setup(const double* E, const double* Mt, const uint8_t* eflag, con
   if (n_aff > dA_cap) {
     if (dA64) (void)hipFree(dA64);
     if (dA) (void)hipFree(dA);
-    if (corr) (void)hipFree(corr);
     dA64 = nullptr;
     dA = nullptr;
-    corr = nullptr;
     dA_cap = n_aff;
     ET_CHECK_HIP(hipMalloc(&dA64, (size_t)dA_cap * 243 * sizeof(double)));
     if (mixed) ET_CHECK_HIP(hipMalloc(&dA, (size_t)dA_cap * 243 * sizeof(float)));
-    ET_CHECK_HIP(hipMalloc(&corr, (size_t)dA_cap * 3 * sizeof(double)));
   }
   l1_delta_kernel<<<grid_for((int64_t)n_aff * 27), kBlock, 0, s>>>(g0, g1, Mt, E, S1, alist, n_aff, dA64);
   ET_CHECK_LAUNCH();
@@ -471,13 +473,16 @@ int Level1MF::apply(int mode, const T* x, const T* b, const T* Dinv, const doubl
   const T* dAt = mixed ? (const T*)dA : (const T*)dA64;
   T* lo = (T*)scratch;
   T* top = lo + 12 * g1.n_elem();
-  T* cr = (T*)corr;
-  const int eb = (int)grid_for(g1.n_elem());
-  const int cb = n_aff > 0 ? (int)grid_for((int64_t)n_aff * 3) : 0;
-  l1_elem_kernel<T><<<eb + cb, 256, 0, s>>>(g1, (const T*)cf, (const T*)Eh, x, lo, top, cb, alist, n_aff, dAt, cr);
+  l1_elem_kernel<T><<<grid_for(g1.n_elem()), 256, 0, s>>>(g1, (const T*)cf, (const T*)Eh, x, lo, top);
   const unsigned gn = grid_for(g1.n_nodes());
-#define ET_L1(M)                                                                                          \
-  l1_node_kernel<T, M><<<gn, 256, 0, s>>>(g1, lo, top, aidx, n_aff, cr, x, b, Dinv, omega, wscale, xadd, y)
+  const unsigned gf = grid_for((int64_t)n_aff * 3);
+#define ET_L1(M)                                                                                     \
+  do {                                                                                               \
+    l1_node_kernel<T, M><<<gn, 256, 0, s>>>(g1, lo, top, aidx, n_aff, dAt, x, b, Dinv, omega, wscale, \
+                                            xadd, y);                                                \
+    if (n_aff > 0)                                                                                   \
+      l1_fix_kernel<T, M><<<gf, 256, 0, s>>>(g1, alist, n_aff, dAt, x, b, Dinv, omega, wscale, xadd, y); \
+  } while (0)
   switch (mode) {
     case 0: ET_L1(0); break;
     case 1: ET_L1(1); break;
@@ -494,10 +499,10 @@ template int Level1MF::apply<double>(int, const double*, const double*, const do
                                      const double*, double*, hipStream_t) const;
 
 void Level1MF::release() {
-  for (void* p : {cf, Eh, scratch, (void*)special, (void*)aidx, (void*)alist, (void*)acount, (void*)dA64, dA, corr})
+  for (void* p : {cf, Eh, scratch, (void*)special, (void*)aidx, (void*)alist, (void*)acount, (void*)dA64, dA})
     if (p) (void)hipFree(p);
   if (acount_h) (void)hipHostFree(acount_h);
-  cf = Eh = scratch = dA = corr = nullptr;
+  cf = Eh = scratch = dA = nullptr;
   special = nullptr;
   aidx = alist = acount = acount_h = nullptr;
   dA64 = nullptr;

@@ -21,7 +21,6 @@ struct Level1MF {
   int n_aff = 0;
   double* dA64 = nullptr;     // correction blocks [27][3][3][n_aff] (FP64)
   void* dA = nullptr;         // the same in working precision
-  void* corr = nullptr;       // correction products dA x [3][n_aff] of one application (working precision)
   int dA_cap = 0;
 
   int init(const Grid& fine, const double* K0_host, bool mixed_, hipStream_t s);
