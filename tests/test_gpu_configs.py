@@ -217,7 +217,7 @@ def test_c4_gradient_central_differences_128x64x64():
 
 
 def test_fp64_trajectory_3d_50_iterations():
-    """North-star parity bar in FP64: compliance after 50 OC iterations within 1e-6 of the
+    """FP64 parity over a whole run: compliance after 50 OC iterations within 1e-6 of the
     oracle (3-D cantilever small enough for the oracle's 50 solves)."""
     import edgeto_amd as ET
     nel = (24, 12, 12)
