@@ -53,7 +53,7 @@ __global__ void cg_init_kernel(const double* __restrict__ part_bb, const double*
     st->rho_old = 0.0;
     st->iters = 0;
     st->first = 1;
-    // |b| = 0 returns x = b at once (cupyx); |r| <= atol before any step also stops
+    // |r| <= atol before any step stops at once (|b| = 0 is handled by the host)
     st->stop = (bb == 0.0 || rr <= st->atol2) ? 1 : 0;
   }
 }
@@ -193,6 +193,19 @@ int cg_solve_op(const KrylovOp& op, const double* b, double* x, double tol, int 
   double* part_b = c.take<double>(kRedBlocks);
   CgState* st = c.take<CgState>(1);
   double* scal = c.take<double>(4);  // [0] |b|^2, [1] |b - A x|^2 after the loop
+  {
+    // |b| = 0: x = b = 0 at once, as cupyx returns
+    ET_TRY(dot(n, b, b, part_a, scal, s));
+    double bb0 = 0.0;
+    ET_CHECK_HIP(hipMemcpyAsync(&bb0, scal, sizeof(double), hipMemcpyDeviceToHost, s));
+    ET_CHECK_HIP(hipStreamSynchronize(s));
+    if (bb0 == 0.0) {
+      ET_CHECK_HIP(hipMemsetAsync(x, 0, n * sizeof(double), s));
+      if (iters) *iters = 0;
+      if (residual) *residual = 0.0;
+      return ET_OK;
+    }
+  }
   // r = b - A x, |b|^2, |r|^2
   ET_TRY(op.apply(1, x, b, r, s));
   ET_TRY(dot_partials(n, b, b, part_a, s));
@@ -346,6 +359,13 @@ int gmres_solve_op(const KrylovOp& op, const double* b, double* x, double tol, i
   double bnorm2 = 0.0;
   ET_CHECK_HIP(hipMemcpyAsync(&bnorm2, sc, sizeof(double), hipMemcpyDeviceToHost, s));
   ET_CHECK_HIP(hipStreamSynchronize(s));
+  if (bnorm2 == 0.0) {
+    // |b| = 0: x = b = 0 at once, as cupyx returns
+    ET_CHECK_HIP(hipMemsetAsync(x, 0, n * sizeof(double), s));
+    if (iters) *iters = 0;
+    if (residual) *residual = 0.0;
+    return ET_OK;
+  }
   const double bnorm = std::sqrt(bnorm2);
   const double atol = tol * bnorm;
   std::vector<double> H((size_t)(m + 1) * m), y(m);

@@ -928,6 +928,19 @@ int MGSolver::pcg(MGHierarchy<T>& h, const double* b, double* x, double tol, int
   // r = b - A x ; |b|^2
   ET_TRY(apply_A64(x, r, OP_RES, b));
   ET_TRY(dot(n_dof, b, b, partial, sh.scal + SC_BB, s));
+  {
+    // b = 0: the solution is x = 0 (the reference's relative test divides by
+    // |b| = 0 and its iterates become NaN); one host read per solve
+    double bb = 0.0;
+    ET_CHECK_HIP(hipMemcpyAsync(&bb, sh.scal + SC_BB, sizeof(double), hipMemcpyDeviceToHost, s));
+    ET_CHECK_HIP(hipStreamSynchronize(s));
+    if (bb == 0.0) {
+      ET_CHECK_HIP(hipMemsetAsync(x, 0, n_dof * sizeof(double), s));
+      if (iters) *iters = 0;
+      if (residual) *residual = 0.0;
+      return ET_OK;
+    }
+  }
   const T* z = nullptr;
   ET_TRY(precondition(h, false, &z));
   prof.begin(ET_PROF_PCG_VEC, s);
