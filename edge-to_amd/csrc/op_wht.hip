@@ -136,16 +136,6 @@ __device__ inline double wht_shr(double v) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-__device__ inline float wht_uniform(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
-}
-__device__ inline double wht_uniform(double v) {
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
-  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
-  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-
 // 4-point transform over (hx, hz): F[px + 2 pz] = sum (-1)^(px hx + pz hz) u[hx][hz]
 template <typename T>
 __device__ inline void wht_face(const T u00, const T u10, const T u01, const T u11, T* F) {
