@@ -43,6 +43,9 @@ class MultiGrid(Solver):
                  low_level_tol=1e-8, low_level_maxiter=5000, min_omega=0.4, omega_boost=1.06,
                  precision="fp64", strict=False):
         super().__init__()
+        # the reference's checks, in its order (solvers/_multigrid_cuda.py:44-54)
+        if coarse_solver in ("splu", "spsolve") and matrix_free:
+            raise ValueError("Matrix free is not supported with splu solver use cg instead.")
         if coarse_solver not in ("cg", "gmres", "splu", "spsolve", "cholesky"):
             raise ValueError("Coarse solver not recognized.")
         if matrix_free:

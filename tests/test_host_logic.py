@@ -1,6 +1,7 @@
 """Host-side logic of the package that needs no GPU: the batched engine's per-problem GMG
 depth rule and the seeded problem generator."""
 import numpy as np
+import pytest
 
 
 def _batch():
@@ -40,3 +41,16 @@ def test_random_problem_is_seeded_and_well_formed():
         assert 1 <= len(p["loads"]) <= 4
         assert not np.intersect1d(p["loads"], p["clamp"]).size
         assert np.allclose(np.linalg.norm(p["dirs"], axis=1), 1.0)
+
+
+def test_multigrid_argument_errors_in_reference_order():
+    """CuMultiGrid's argument checks run before any device work and in the reference's order
+    (solvers/_multigrid_cuda.py:44-54): matrix_free with a direct coarse solver is a
+    ValueError, an unknown coarse solver a ValueError, matrix_free otherwise not implemented."""
+    from edgeto_amd.solvers import MultiGrid
+    with pytest.raises(ValueError, match="splu"):
+        MultiGrid(None, None, matrix_free=True)
+    with pytest.raises(ValueError, match="not recognized"):
+        MultiGrid(None, None, coarse_solver="lu")
+    with pytest.raises(NotImplementedError):
+        MultiGrid(None, None, coarse_solver="cg", matrix_free=True)
