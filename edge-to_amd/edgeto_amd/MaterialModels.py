@@ -4,11 +4,34 @@ Mirrors EDGETO/MaterialModels.py: MaterialModel (:3-28) and SingleMaterial
 (:31-99).  The elementwise maps run in the library (et_simp_forward_f64);
 the OC multiplier and the volume test are evaluated inside the device OC
 update (et_oc_update_f64) rather than as separate host-synchronising calls.
+evaluate_constraint / ocP are kept for callers that drive their own OC loop:
+they take device tensors (or host arrays, moved to the device) and compute
+with torch on the device.
 """
 import numpy as np
 import torch
 
 from . import _lib
+
+
+def _dev_args(*xs):
+    """float64 tensors on the device of the first tensor argument (else cuda:0)."""
+    dev = next((x.device for x in xs if isinstance(x, torch.Tensor)), None)
+    dev = torch.device("cuda") if dev is None else dev
+    return [torch.as_tensor(x, dtype=torch.float64, device=dev) for x in xs]
+
+
+def _oc_product(df, dg, rho, clamp):
+    """rho * nan_to_num(sqrt(-df / dg[:, None])), negative ratios giving 0
+    (MaterialModels.py:83-90, :217-225); all zero -> 1e-3 everywhere."""
+    df, dg, rho = _dev_args(df, dg, rho)
+    q = -df / dg.reshape(-1, 1)
+    if clamp:
+        q = torch.clamp(q, min=0.0)
+    out = rho * torch.nan_to_num(torch.sqrt(q), nan=0.0)
+    if float(out.abs().sum()) == 0.0:
+        out = torch.full_like(out, 1e-3)
+    return out
 
 
 class MaterialModel:
@@ -19,6 +42,12 @@ class MaterialModel:
 
     def __call__(self, rho, iteration, **kwargs):
         return self.__forward__(rho, iteration, **kwargs)
+
+    def evaluate_constraint(self, rho, dg, vol, **kwargs):
+        return self.__con__(rho, dg, vol, **kwargs)
+
+    def ocP(self, df, dg, rho, **kwargs):
+        return self.__ocP__(df, dg, rho, **kwargs)
 
     def grad(self, rho, iteration, **kwargs):
         return self.__backward__(rho, iteration, **kwargs)
@@ -63,6 +92,15 @@ class SingleMaterial(MaterialModel):
         _lib.call("et_simp_forward_f64", clipped.numel(), _lib.f64(clipped), pen - 1.0, 0.0, 0,
                   _lib.f64(E, clipped.numel()), _lib.stream())
         return E * pen
+
+    def __con__(self, rho, dg, vol, **_):
+        """[sum(rho * dg) / vol <= volume_fraction] as a host bool array (MaterialModels.py:75-81)."""
+        rho, dg = _dev_args(rho, dg)
+        V = float((rho.reshape(-1) * dg.reshape(-1)).sum())
+        return np.array([V / vol <= self.volume_fraction])
+
+    def __ocP__(self, df, dg, rho, **_):
+        return _oc_product(df, dg, rho, clamp=True)
 
     def __initvars__(self, nel, dtype=torch.float64, **_):
         return torch.ones(nel, dtype=torch.float64, device=torch.device("cuda"))
@@ -129,14 +167,18 @@ class PenalizedMultiMaterial(MaterialModel):
         return self._call(rho, iteration, 2)
 
     def __con__(self, rho, dg, vol, **_):
-        """Constraint flags per material (host booleans; evaluated on the device)."""
-        rho = rho.reshape(-1, self.n_material)
-        A = float(dg.reshape(-1)[0]) if isinstance(dg, torch.Tensor) else float(np.asarray(dg).reshape(-1)[0])
+        """Constraint flags per material as host booleans (MaterialModels.py:203-215): each
+        material's volume sum(rho_k * dg) / vol against its mass, or with rho_mat the total
+        mass against the one budget.  dg is one measure per element or one shared value."""
+        rho, dg = _dev_args(rho, dg)
+        V = (rho.reshape(-1, self.n_material) * dg.reshape(-1, 1)).sum(dim=0).cpu().numpy()
         if self.rho_mat is None:
-            V = (rho.sum(dim=0) * A).cpu().numpy()
             return (V / vol - self.mass.reshape(-1)) <= 0
-        V = float((rho.sum(dim=0).cpu().numpy() * A) @ self.rho_mat.reshape(-1))
+        V = float(V @ self.rho_mat.reshape(-1))
         return np.ones(self.n_material, dtype=bool) * (V / vol <= self.mass.reshape(-1)[0])
+
+    def __ocP__(self, df, dg, rho, **_):
+        return _oc_product(df, dg, rho, clamp=False)
 
     def __initvars__(self, nel, dtype=torch.float64, **_):
         return torch.ones((nel, self.n_material), dtype=torch.float64, device=torch.device("cuda"))

@@ -22,14 +22,13 @@ consistently (the reference renumbers elements_flat only and then evaluates the
 element matrices with the old numbering, geom/_mesh.py:172-190).
 """
 import ctypes
-import warnings
 
 import numpy as np
 import torch
 
 from . import _lib
 from .geom import FilterKernel, Mesh, _device
-from .kernels import StiffnessKernel
+from .kernels import CSRAssembly, StiffnessKernel
 from .physics import AutoStiffness
 
 
@@ -144,7 +143,7 @@ class GeneralMesh(Mesh):
 CuGeneralMesh = GeneralMesh
 
 
-class _UnstructuredKernel(StiffnessKernel):
+class _UnstructuredKernel(CSRAssembly, StiffnessKernel):
     """Shared part of the Uniform / General stiffness kernels (kernels/_cuda.py:305-815)."""
 
     def __init__(self, mesh):
@@ -218,15 +217,13 @@ class _UnstructuredKernel(StiffnessKernel):
         return self._matvec(rho, vec)
 
     def dot(self, rhs):
-        if not self.has_rho:
-            raise ValueError("Rho has not been set. dot works only after setting rho.")
-        if not isinstance(rhs, torch.Tensor) or not rhs.is_cuda:
-            raise NotImplementedError("Only device tensors are supported as vectors.")
-        if rhs.shape[0] != self.shape[0]:
-            raise ValueError("Shape of the input vector does not match the number of nodes and dof.")
-        if rhs.dtype != torch.float64:
-            raise ValueError("vectors must be float64")
-        return self._matvec(self.rho, rhs)
+        return self._dot_checked(rhs)
+
+    def _n_elements(self):
+        return self.n_el
+
+    def _assembly_mesh(self):
+        return self._um
 
     def diagonal(self, rho=None):
         if rho is None and not self.has_rho:
@@ -253,35 +250,6 @@ class _UnstructuredKernel(StiffnessKernel):
         _lib.call("et_usensitivity_f64", ctypes.byref(self._um), _lib.f64(U, self.shape[0]),
                   _lib.f64(xt, self.n_el), float(penal), float(void), _lib.f64(out, self.n_el), _lib.stream())
         return out
-
-    def _pattern(self):
-        """CSR row offsets and columns of the assembled operator (built once, et_umesh_pattern)."""
-        if getattr(self, "_csr", None) is None:
-            dev = self.constraints.device
-            rp = torch.empty(self.shape[0] + 1, dtype=torch.int32, device=dev)
-            nnz = ctypes.c_int64(0)
-            _lib.call("et_umesh_pattern", ctypes.byref(self._um), _lib.i32(rp), None, 0, ctypes.byref(nnz),
-                      _lib.stream())
-            cols = torch.empty(int(nnz.value), dtype=torch.int32, device=dev)
-            _lib.call("et_umesh_pattern", ctypes.byref(self._um), _lib.i32(rp), _lib.i32(cols), int(nnz.value),
-                      ctypes.byref(nnz), _lib.stream())
-            self._csr = (rp, cols)
-        return self._csr
-
-    def construct(self, rho):
-        """The assembled operator A_hat = M K(rho) M + I_c as a device CSR matrix
-        (kernels/_cuda.py:407-420); the sparsity pattern is built on the first call."""
-        rp, cols = self._pattern()
-        rho = rho.reshape(-1).contiguous()
-        vals = torch.empty(cols.numel(), dtype=torch.float64, device=cols.device)
-        _lib.call("et_uassemble_f64", ctypes.byref(self._um), _lib.f64(rho, self.n_el), self._cons_ptr(),
-                  _lib.i32(rp), _lib.i32(cols), _lib.f64(vals), _lib.stream())
-        with warnings.catch_warnings():
-            warnings.simplefilter("ignore", UserWarning)  # torch marks sparse CSR tensors as beta
-            self.CSR = torch.sparse_csr_tensor(rp, cols, vals, size=self.shape)
-        self._csr_vals = vals
-        self.has_been_constructed = True
-        return self.CSR
 
     def __matmul__(self, rhs):
         return self.dot(rhs)
