@@ -387,11 +387,88 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     for (int q = 0; q < 3; ++q) Sc[((int64_t)(o * 3 + r) * 3 + q) * Nc + c] = blk[r][q];
 }
 
-int galerkin(const Grid& gf, int dof, const double* Sf, double* Sc, hipStream_t s) {
+// One axis of the 3-D Galerkin product in separable form.  The input is a
+// block stencil (27 offsets of 3 x 3 blocks, [(o * 3 + r) * 3 + q] * N + node)
+// on a node grid that is fine along axis AX (and coarse along the axes already
+// reduced); the output has axis AX coarse.  Output node c, offset o: the sum
+// over the (d, e) pairs of o's component along AX (gal_pair) of
+// pw(d) pw(e) in[c with c_AX -> 2 c_AX + d][o with o_AX -> 2 o_AX + e - d]
+// over pairs whose fine nodes 2 c_AX + d and 2 (c_AX + o_AX) + e lie in the
+// grid, times 1 / (tw(c_AX) tw(c_AX + o_AX)) (tw is a product of per-axis
+// factors); zero where c_AX + o_AX is outside the coarse grid.  The passes z,
+// y, x give galerkin3_kernel's sums with each fine stencil entry read once from
+// memory instead of once per coarse pair that needs it.  Thread per (output
+// node, offset): 9 sums of at most 7 terms, all loads unconditional.
+__device__ __forceinline__ double tw_axis(int c, int cn) { return (c == 0 || c == cn - 1) ? 1.5 : 2.0; }
+
+template <int AX>
+__global__ __launch_bounds__(256) void galerkin_axis_kernel(int inx, int iny, int inz, int onx, int ony, int onz,
+                                                            const double* __restrict__ in,
+                                                            double* __restrict__ out) {
+  const int64_t No = (int64_t)onx * ony * onz, Ni = (int64_t)inx * iny * inz;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= No) return;
+  const int o = (int)blockIdx.y;
+  const int oo[3] = {o / 9 - 1, (o / 3) % 3 - 1, o % 3 - 1};
+  const int64_t u = t / onz;
+  const int cc[3] = {(int)(u % onx), (int)(u / onx), (int)(t % onz)};
+  const int ca = cc[AX], oa = oo[AX];
+  const int cn = AX == 0 ? onx : (AX == 1 ? ony : onz);
+  const int fn = AX == 0 ? inx : (AX == 1 ? iny : inz);
+  double blk[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) blk[q] = 0.0;
+  if (ca + oa >= 0 && ca + oa < cn) {
+    const int np = gal_npairs(oa);  // uniform per workgroup (o = grid y)
+#pragma unroll
+    for (int p = 0; p < 7; ++p) {
+      if (p < np) {
+        int d, e;
+        gal_pair(oa, p, d, e);
+        const int f = 2 * ca + d, g = 2 * (ca + oa) + e;
+        const double w = (f >= 0 && f < fn && g >= 0 && g < fn) ? pw(d) * pw(e) : 0.0;
+        int nn[3] = {cc[0], cc[1], cc[2]};
+        nn[AX] = min(max(f, 0), fn - 1);
+        int ss[3] = {oo[0], oo[1], oo[2]};
+        ss[AX] = 2 * oa + e - d;
+        const int64_t ni = ((int64_t)nn[1] * inx + nn[0]) * inz + nn[2];
+        const int so = (ss[0] + 1) * 9 + (ss[1] + 1) * 3 + (ss[2] + 1);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) blk[q] += w * in[(int64_t)(so * 9 + q) * Ni + ni];
+      }
+    }
+    const double inv = 1.0 / (tw_axis(ca, cn) * tw_axis(ca + oa, cn));
+#pragma unroll
+    for (int q = 0; q < 9; ++q) blk[q] *= inv;
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) out[(int64_t)(o * 9 + q) * No + t] = blk[q];
+}
+
+int64_t galerkin_work_doubles(const Grid& gf) {
+  if (gf.dim != 3) return 0;
+  const Grid gc = gf.coarse();
+  const int64_t fx = gf.nx(), fy = gf.ny(), cy = gc.ny(), cz = gc.nz();
+  return 27 * 9 * (fx * fy * cz + fx * cy * cz);
+}
+
+int galerkin(const Grid& gf, int dof, const double* Sf, double* Sc, hipStream_t s, double* work) {
   const Grid gc = gf.coarse();
   const int NO = gf.dim == 3 ? 27 : 9;
   const unsigned grid = grid_for(gc.n_nodes() * NO);
-  if (gf.dim == 3) {
+  if (gf.dim == 3 && work != nullptr) {
+    ET_REQUIRE(dof == 3, "3-D Galerkin expects dof 3");
+    const int fx = gf.nx(), fy = gf.ny(), fz = gf.nz(), cx = gc.nx(), cy = gc.ny(), cz = gc.nz();
+    double* T1 = work;                                  // (fx, fy, cz)
+    double* T2 = work + (int64_t)27 * 9 * fx * fy * cz;  // (fx, cy, cz)
+    galerkin_axis_kernel<2><<<dim3(grid_for((int64_t)fx * fy * cz), 27), kBlock, 0, s>>>(fx, fy, fz, fx, fy, cz,
+                                                                                         Sf, T1);
+    galerkin_axis_kernel<1><<<dim3(grid_for((int64_t)fx * cy * cz), 27), kBlock, 0, s>>>(fx, fy, cz, fx, cy, cz,
+                                                                                         T1, T2);
+    galerkin_axis_kernel<0><<<dim3(grid_for((int64_t)cx * cy * cz), 27), kBlock, 0, s>>>(fx, cy, cz, cx, cy, cz,
+                                                                                         T2, Sc);
+    (void)grid;
+  } else if (gf.dim == 3) {
     ET_REQUIRE(dof == 3, "3-D Galerkin expects dof 3");
     const dim3 g3(grid_for(gc.n_nodes()), 9);
     galerkin3_kernel<-1><<<g3, kBlock, 0, s>>>(gf, gc, Sf, Sc);

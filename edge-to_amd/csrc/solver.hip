@@ -10,6 +10,7 @@
 // memory; the host reads |r|^2 once per iteration for the convergence test.
 // precision 1 runs the V-cycle in FP32 (operators cast from the FP64
 // hierarchy) inside the FP64 outer loop.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -745,6 +746,18 @@ int MGSolver::create(const et_mg_params& p, const double* K0_host, hipStream_t s
       g = g.coarse();
     }
   }
+  {
+    // scratch of the three-pass Galerkin products (the largest of levels built from a stencil);
+    // above 4 GiB the one-pass kernels run instead
+    int64_t need = 0;
+    Grid g = g0;
+    for (int l = 1; l <= p.n_level; ++l) {
+      if (!(l == 1 && elem_galerkin)) need = std::max(need, galerkin_work_doubles(g));
+      g = g.coarse();
+    }
+    if (need > 0 && need * (int64_t)sizeof(double) <= ((int64_t)1 << 32))
+      ET_CHECK_HIP(hipMalloc(&gal_work, (size_t)need * sizeof(double)));
+  }
   use_l1 = elem_galerkin && get_level1_mf() && level1_supported(g0, p.n_level) && (k0_flags & K0S_LEVEL1);
   if (use_l1) ET_TRY(l1.init(g0, K0_host, p.precision == 1, s));
   sh.l1 = use_l1 ? &l1 : nullptr;
@@ -776,7 +789,7 @@ void MGSolver::destroy() {
   for (double* v : S64)
     if (v) (void)hipFree(v);
   for (void* v : {(void*)sh.K0d, (void*)sh.K0f, (void*)sh.scal, (void*)partial, (void*)r, (void*)z,
-                  (void*)p_, (void*)q, (void*)diag64, (void*)sh.Ef, (void*)Mt, (void*)eflag})
+                  (void*)p_, (void*)q, (void*)diag64, (void*)sh.Ef, (void*)Mt, (void*)eflag, (void*)gal_work})
     if (v) (void)hipFree(v);
   if (ring) (void)hipHostFree(ring);
   for (auto& e : ring_ev)
@@ -804,7 +817,7 @@ int MGSolver::setup_levels(MGHierarchy<T>& h) {
       ET_TRY(galerkin_from_elements(g0, sh.K0d, Mt, sh.Ed, sh.cons, eflag, S64[1], s));
       if (use_l1) ET_TRY(l1.setup(sh.Ed, Mt, eflag, S64[1], s));
     } else
-      ET_TRY(galerkin(g, dof, S64[l - 1], S64[l], s));
+      ET_TRY(galerkin(g, dof, S64[l - 1], S64[l], s, gal_work));
     g = g.coarse();
     MGLevel<T>& lev = h.lv[l];
     const int64_t N = g.n_nodes();

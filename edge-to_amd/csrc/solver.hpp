@@ -128,6 +128,7 @@ struct MGSolver {
   bool elem_galerkin = false;  // level 1 built from elements (level-0 stencil not formed)
   double* Mt = nullptr;        // child tables W_t^T K0 W_t
   uint8_t* eflag = nullptr;    // per-element "touches a constrained DOF"
+  double* gal_work = nullptr;  // intermediates of the three-pass 3-D Galerkin product (null: one-pass)
   Level1MF l1;                 // matrix-free level 1 (3-D, even counts, n_level >= 2)
   bool use_l1 = false;
   int k0_flags = 0;            // k0_structure() of the element matrix (fast 3-D kernels only if it allows)

@@ -155,7 +155,10 @@ template <typename T>
 int prolong_op(const Grid& g, int dof, const T* vc, T* vf, hipStream_t s);
 
 // stencils (double)
-int galerkin(const Grid& g, int dof, const double* Sf, double* Sc, hipStream_t s);
+// work: galerkin_work_doubles(g) doubles for the three-pass 3-D product, or
+// nullptr for the one-pass kernels (no scratch)
+int galerkin(const Grid& g, int dof, const double* Sf, double* Sc, hipStream_t s, double* work = nullptr);
+int64_t galerkin_work_doubles(const Grid& g);
 template <typename T>
 int stencil_apply(const Grid& g, int dof, const T* S, const T* x, T* y, hipStream_t s);
 
