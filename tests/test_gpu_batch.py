@@ -36,7 +36,7 @@ def _small_problem(i):
                 vf=float(rng.uniform(0.2, 0.6)), rmin=float(rng.uniform(1.2, 3.0)), penal=3.0)
 
 
-def _single(ET, pb, n_iter, n_level):
+def _single(ET, pb, n_iter, n_level, precision="mixed"):
     dim, nel = pb["dim"], pb["nel"]
     stiff = lambda x0: ET.auto_stiffness(x0, 1.0, 0.3)
     if dim == 2:
@@ -47,7 +47,7 @@ def _single(ET, pb, n_iter, n_level):
         filt = ET.CuStructuredFilter3D(mesh, pb["rmin"])
     kernel = ET.CuStructuredStiffnessKernel(mesh)
     mat = ET.SingleMaterial(void=1e-9, penalty=3.0, volume_fraction=pb["vf"])
-    solver = ET.CuMultiGrid(mesh, kernel, tol=1e-8, n_smooth=2, n_level=n_level, precision="mixed")
+    solver = ET.CuMultiGrid(mesh, kernel, tol=1e-8, n_smooth=2, n_level=n_level, precision=precision)
     tp = ET.CuTopOpt(mesh, mat, kernel, solver, filt, max_iter=n_iter, ch_tol=0, fun_tol=0, verbose=False)
     tp.add_BC_nodal(pb["clamp"], np.ones((1, dim), dtype=np.int64))
     tp.add_F_nodal(pb["loads"], pb["dirs"])
@@ -57,15 +57,16 @@ def _single(ET, pb, n_iter, n_level):
     return comp, pcg, x.cpu().numpy()
 
 
-def test_batch_matches_single_problem_runs():
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_batch_matches_single_problem_runs(precision):
     import edgeto_amd as ET
     n_iter = 4
     problems = [_small_problem(i) for i in range(8)]
-    batch = ET.BatchTopOpt(problems, precision="mixed", tiles={})
+    batch = ET.BatchTopOpt(problems, precision=precision, tiles={})
     out = batch.run(n_iter, n_workers=4)
     assert out["wall_ms"] > 0
     for i, pb in enumerate(problems):
-        comp, pcg, x = _single(ET, pb, n_iter, batch.levels[i])
+        comp, pcg, x = _single(ET, pb, n_iter, batch.levels[i], precision)
         got = out["problems"][i]
         rel = np.abs(got["compliance"] - comp) / np.abs(comp)
         assert rel.max() <= 1e-10, (i, pb["nel"], rel.max())
