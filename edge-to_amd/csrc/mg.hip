@@ -398,7 +398,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // factors); zero where c_AX + o_AX is outside the coarse grid.  The passes z,
 // y, x give galerkin3_kernel's sums with each fine stencil entry read once from
 // memory instead of once per coarse pair that needs it.  Thread per (output
-// node, offset): 9 sums of at most 7 terms, all loads unconditional.
+// node, offsets on the other two axes): 9 input blocks, 27 sums.
 __device__ __forceinline__ double tw_axis(int c, int cn) { return (c == 0 || c == cn - 1) ? 1.5 : 2.0; }
 
 template <int AX>
@@ -408,41 +408,66 @@ __global__ __launch_bounds__(256) void galerkin_axis_kernel(int inx, int iny, in
   const int64_t No = (int64_t)onx * ony * onz, Ni = (int64_t)inx * iny * inz;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= No) return;
-  const int o = (int)blockIdx.y;
-  const int oo[3] = {o / 9 - 1, (o / 3) % 3 - 1, o % 3 - 1};
+  // grid y: the offsets on the two other axes; this thread forms all three
+  // offsets o_AX = -1, 0, 1, so each input block it needs is loaded once
+  const int ob = (int)blockIdx.y;
+  int oo[3];
+  {
+    const int u0 = ob / 3 - 1, u1 = ob % 3 - 1;  // the other axes in x, y, z order
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) oo[a] = a == AX ? 0 : (k++ == 0 ? u0 : u1);
+  }
   const int64_t u = t / onz;
   const int cc[3] = {(int)(u % onx), (int)(u / onx), (int)(t % onz)};
-  const int ca = cc[AX], oa = oo[AX];
+  const int ca = cc[AX];
   const int cn = AX == 0 ? onx : (AX == 1 ? ony : onz);
   const int fn = AX == 0 ? inx : (AX == 1 ? iny : inz);
-  double blk[9];
+  double acc[3][9];
 #pragma unroll
-  for (int q = 0; q < 9; ++q) blk[q] = 0.0;
-  if (ca + oa >= 0 && ca + oa < cn) {
-    const int np = gal_npairs(oa);  // uniform per workgroup (o = grid y)
+  for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int p = 0; p < 7; ++p) {
-      if (p < np) {
-        int d, e;
-        gal_pair(oa, p, d, e);
-        const int f = 2 * ca + d, g = 2 * (ca + oa) + e;
-        const double w = (f >= 0 && f < fn && g >= 0 && g < fn) ? pw(d) * pw(e) : 0.0;
-        int nn[3] = {cc[0], cc[1], cc[2]};
-        nn[AX] = min(max(f, 0), fn - 1);
-        int ss[3] = {oo[0], oo[1], oo[2]};
-        ss[AX] = 2 * oa + e - d;
-        const int64_t ni = ((int64_t)nn[1] * inx + nn[0]) * inz + nn[2];
-        const int so = (ss[0] + 1) * 9 + (ss[1] + 1) * 3 + (ss[2] + 1);
+    for (int q = 0; q < 9; ++q) acc[a][q] = 0.0;
+  // fine positions f = 2 c + d and fine offsets s along AX; output offset oa
+  // takes the pair (d, e = s - 2 oa + d) when e is in -1..1.  Loop order d, s
+  // sums each output's pairs in gal_pair's order.  One fine position at a time
+  // (its three blocks, 27 loads in flight) keeps the registers for 4 waves per SIMD.
+#pragma unroll 1
+  for (int d = -1; d <= 1; ++d) {
+    const int f = 2 * ca + d;
+    const bool fin = f >= 0 && f < fn;
+    int nn[3] = {cc[0], cc[1], cc[2]};
+    nn[AX] = min(max(f, 0), fn - 1);
+    const int64_t ni = ((int64_t)nn[1] * inx + nn[0]) * inz + nn[2];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) blk[q] += w * in[(int64_t)(so * 9 + q) * Ni + ni];
+    for (int sa = -1; sa <= 1; ++sa) {
+      int ss[3] = {oo[0], oo[1], oo[2]};
+      ss[AX] = sa;
+      const int so = (ss[0] + 1) * 9 + (ss[1] + 1) * 3 + (ss[2] + 1);
+      double v[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) v[q] = in[(int64_t)(so * 9 + q) * Ni + ni];
+#pragma unroll
+      for (int oa = -1; oa <= 1; ++oa) {
+        const int e = sa - 2 * oa + d;
+        if (e < -1 || e > 1) continue;  // uniform: d, sa, oa are the same in every lane
+        const int g = 2 * (ca + oa) + e;
+        const double w = (fin && g >= 0 && g < fn) ? pw(d) * pw(e) : 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[oa + 1][q] += w * v[q];
       }
     }
-    const double inv = 1.0 / (tw_axis(ca, cn) * tw_axis(ca + oa, cn));
-#pragma unroll
-    for (int q = 0; q < 9; ++q) blk[q] *= inv;
   }
 #pragma unroll
-  for (int q = 0; q < 9; ++q) out[(int64_t)(o * 9 + q) * No + t] = blk[q];
+  for (int oa = -1; oa <= 1; ++oa) {
+    const bool tin = ca + oa >= 0 && ca + oa < cn;
+    const double inv = tin ? 1.0 / (tw_axis(ca, cn) * tw_axis(ca + oa, cn)) : 0.0;
+    int os[3] = {oo[0], oo[1], oo[2]};
+    os[AX] = oa;
+    const int o = (os[0] + 1) * 9 + (os[1] + 1) * 3 + (os[2] + 1);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) out[(int64_t)(o * 9 + q) * No + t] = tin ? acc[oa + 1][q] * inv : 0.0;
+  }
 }
 
 int64_t galerkin_work_doubles(const Grid& gf) {
@@ -461,12 +486,12 @@ int galerkin(const Grid& gf, int dof, const double* Sf, double* Sc, hipStream_t 
     const int fx = gf.nx(), fy = gf.ny(), fz = gf.nz(), cx = gc.nx(), cy = gc.ny(), cz = gc.nz();
     double* T1 = work;                                  // (fx, fy, cz)
     double* T2 = work + (int64_t)27 * 9 * fx * fy * cz;  // (fx, cy, cz)
-    galerkin_axis_kernel<2><<<dim3(grid_for((int64_t)fx * fy * cz), 27), kBlock, 0, s>>>(fx, fy, fz, fx, fy, cz,
-                                                                                         Sf, T1);
-    galerkin_axis_kernel<1><<<dim3(grid_for((int64_t)fx * cy * cz), 27), kBlock, 0, s>>>(fx, fy, cz, fx, cy, cz,
-                                                                                         T1, T2);
-    galerkin_axis_kernel<0><<<dim3(grid_for((int64_t)cx * cy * cz), 27), kBlock, 0, s>>>(fx, cy, cz, cx, cy, cz,
-                                                                                         T2, Sc);
+    galerkin_axis_kernel<2><<<dim3(grid_for((int64_t)fx * fy * cz), 9), kBlock, 0, s>>>(fx, fy, fz, fx, fy, cz,
+                                                                                        Sf, T1);
+    galerkin_axis_kernel<1><<<dim3(grid_for((int64_t)fx * cy * cz), 9), kBlock, 0, s>>>(fx, fy, cz, fx, cy, cz,
+                                                                                        T1, T2);
+    galerkin_axis_kernel<0><<<dim3(grid_for((int64_t)cx * cy * cz), 9), kBlock, 0, s>>>(fx, cy, cz, cx, cy, cz,
+                                                                                        T2, Sc);
     (void)grid;
   } else if (gf.dim == 3) {
     ET_REQUIRE(dof == 3, "3-D Galerkin expects dof 3");
