@@ -54,3 +54,26 @@ def test_multigrid_argument_errors_in_reference_order():
         MultiGrid(None, None, coarse_solver="lu")
     with pytest.raises(NotImplementedError):
         MultiGrid(None, None, coarse_solver="cg", matrix_free=True)
+
+
+def test_grid_size_limit_is_an_error_before_any_device_access():
+    """A grid whose DOF count does not fit 32-bit indices (2000^3 elements) is rejected with
+    ET_EINVAL by every grid entry point before it touches memory (host-only: dummy pointers)."""
+    import ctypes
+    from edgeto_amd import _lib
+    L = _lib.lib()
+    nel = (ctypes.c_int * 3)(2000, 2000, 2000)
+    p = ctypes.cast((ctypes.c_double * 8)(), ctypes.c_void_p)
+    q = ctypes.cast((ctypes.c_double * 8)(), ctypes.c_void_p)
+    calls = {
+        "et_matvec_f64": lambda: L.et_matvec_f64(3, nel, p, p, None, p, q, None),
+        "et_residual_f64": lambda: L.et_residual_f64(3, nel, p, p, None, p, p, q, None),
+        "et_jacobi_f64": lambda: L.et_jacobi_f64(3, nel, p, p, None, p, p, p, 0.6, q, None),
+        "et_diagonal_f64": lambda: L.et_diagonal_f64(3, nel, p, p, None, q, None),
+        "et_elem_energy_f64": lambda: L.et_elem_energy_f64(3, nel, p, p, q, None),
+        "et_galerkin_f64": lambda: L.et_galerkin_f64(3, nel, 3, p, q, None),
+        "et_generate_elements": lambda: L.et_generate_elements(3, nel, p, None),
+    }
+    for name, call in calls.items():
+        assert call() == 1, name
+        assert b"32-bit" in L.et_last_error(), (name, L.et_last_error())
