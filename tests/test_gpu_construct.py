@@ -1,7 +1,7 @@
 """construct() of the structured stiffness kernel and the matrix products of the kernels
 (kernels/_cuda.py:158-170, :213-302): the device CSR against the oracle's assembled operator
 (2-D and 3-D grids of several aspect ratios, constraints included), its product against the
-matrix-free operator, dot() of a sparse identity and of a dense block of columns."""
+matrix-free operator, dot() of a sparse identity, of a random sparse block and of a dense block."""
 import numpy as np
 import pytest
 
@@ -104,3 +104,19 @@ def test_general_kernel_matmat_dense():
     assert float((KB - A @ B).abs().max()) <= 1e-13 * float(KB.abs().max())
     for j in range(3):
         assert torch.equal(KB[:, j], kernel.dot(B[:, j].contiguous()))
+
+
+def test_structured_matmat_random_sparse():
+    """dot() of a random sparse CSR block (8 columns, ~5 % dense) with the structured kernel: the
+    sparse product equals the assembled operator times the dense block and the oracle's product."""
+    kernel, mesh, els, E, cons, rng = _kernel(3, (4, 2, 3), (4.0, 2.0, 3.0), 12)
+    n = kernel.shape[0]
+    Bd = rng.standard_normal((n, 8)) * (rng.random((n, 8)) < 0.05)
+    Bd[0, 0] = 1.0  # at least one entry
+    B = torch.tensor(Bd, device="cuda").to_sparse_csr()
+    KB = kernel.dot(B)
+    assert KB.layout == torch.sparse_csr and tuple(KB.shape) == (n, 8)
+    ref = kernel.construct(kernel.rho).to_dense() @ torch.tensor(Bd, device="cuda")
+    assert float((KB.to_dense() - ref).abs().max()) <= 1e-13 * float(ref.abs().max())
+    Ao = oops.assemble(mesh.K_single_host, els, E, cons, n).toarray()
+    assert np.abs(KB.to_dense().cpu().numpy() - Ao @ Bd).max() <= 1e-12 * np.abs(Ao @ Bd).max()
